@@ -1,0 +1,132 @@
+// Shared definitions for the tdscha_amd HIP library (gfx950 / CDNA4).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace tdscha {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define TDS_CHECK_HIP(expr)                                                   \
+  do {                                                                        \
+    hipError_t _e = (expr);                                                   \
+    if (_e != hipSuccess) {                                                   \
+      ::tdscha::set_error(std::string(#expr) + ": " + hipGetErrorString(_e)); \
+      return -1;                                                              \
+    }                                                                         \
+  } while (0)
+
+#define TDS_REQUIRE(cond, code, msg)            \
+  do {                                          \
+    if (!(cond)) {                              \
+      ::tdscha::set_error(msg);                 \
+      return (code);                            \
+    }                                           \
+  } while (0)
+
+// ------------------------------------------------------------ constants
+constexpr double RY_TO_K = 157887.32400374097;
+constexpr int kTile = 128;      // column tile of the weighted outer-product GEMM
+constexpr int kRowTile = 64;    // image rows per workgroup of the weights GEMM
+constexpr int kMaxChains = 8;   // perturbations applied in one batched call
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+inline long round_up_l(long x, long m) { return (x + m - 1) / m * m; }
+
+// MFMA operand vector types
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef _Float16 v8h __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+enum Precision { PREC_FP64 = 0, PREC_FP16X2 = 1, PREC_BF16X2 = 2 };
+
+// Per-precision image storage for the split-precision kernels.
+struct SplitStore {
+  void* Uh = nullptr;  // [Mp, np] hi part of scaled images
+  void* Ul = nullptr;  // [Mp, np] lo part
+  void* Fh = nullptr;
+  void* Fl = nullptr;
+  double* su = nullptr;  // [np] power-of-two column scale of U
+  double* sf = nullptr;  // [np] power-of-two column scale of F
+  double* umax = nullptr;  // [np] max |U| per column (unscaled)
+  double* fmax = nullptr;  // [np] max |F| per column (unscaled)
+  bool ready = false;
+};
+
+// ------------------------------------------------------------- context
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // problem
+  int n = 0;      // modes
+  int np = 0;     // padded modes (multiple of kTile)
+  int N = 0;      // configurations
+  int n_syms = 0;
+  long M = 0;     // images N * n_syms
+  long Mp = 0;    // padded images (multiple of the K split granule)
+  double T = 0;
+  double n_eff = 0;
+  // symmetric image ensemble, device resident
+  double* U = nullptr;      // [Mp, np] S_j x_i
+  double* F = nullptr;      // [Mp, np] S_j y_i
+  double* rho_img = nullptr;// [Mp]
+  double* fy = nullptr;     // [np]
+  double* fpsi = nullptr;   // [np]
+  double* w = nullptr;      // [np]
+  // per-call buffers
+  double* R1 = nullptr;     // [kMaxChains, np]
+  double* Y1 = nullptr;     // [kMaxChains, np, np]
+  double* coef = nullptr;   // [kMaxChains, Mp, 4] (c1, c2, alpha, beta)
+  double* zpart = nullptr;  // [kMaxChains, ksplit, np, np]
+  double* fpart = nullptr;  // [kMaxChains, ksplit, np]
+  double* f_out = nullptr;  // [kMaxChains, np]
+  double* d2v_out = nullptr;// [kMaxChains, np, np]
+  int ksplit = 0;
+  // split precision
+  SplitStore split[3];
+  void* Y1h = nullptr;      // [kMaxChains, np, np] (16-bit)
+  void* Y1l = nullptr;
+  double* ysc = nullptr;    // [kMaxChains, np] column scale of split Y1
+  double* bsc = nullptr;    // [kMaxChains, np] column scale of split B
+  double* cmax = nullptr;   // [kMaxChains, 2] max|c1|, max|c2|/2 (device)
+  // Wigner / harmonic helpers
+  double* pair_ca = nullptr;   // [npairs]
+  double* pair_cb = nullptr;
+  double* pair_sm = nullptr;
+  double* pair_sp = nullptr;
+  double* pair_harm_a = nullptr;
+  double* pair_harm_b = nullptr;
+  int* pair_row = nullptr;
+  int* pair_col = nullptr;
+  // standard representation helpers
+  double* std_coef = nullptr;  // [8, npairs] harmonic 2x2 block coefficients
+  double* yw = nullptr;        // [np] Y_w
+  double* reaw = nullptr;      // [np] ReA_w
+  // gamma-only translational projector
+  int n_cells = 0;
+  int n_cart = 0;
+  double* pols = nullptr;      // [n_cart, n]
+  int* inv_perms = nullptr;    // [n_cells, n_cart]
+  double* tproj = nullptr;     // [n, n]
+  double* cart_tmp = nullptr;  // [kMaxChains, n_cart, n_cart] x2
+  double* cart_tmp2 = nullptr;
+  double* mode_tmp = nullptr;  // [kMaxChains, n, n_cart]
+  // scratch for the anharmonic wrapper
+  double* psi_dev = nullptr;   // [kMaxChains, len_psi] x2
+  double* psi_out_dev = nullptr;
+  double* stage_in = nullptr;  // [kMaxChains, n + n*n] host<->device staging
+  double* stage_out = nullptr;
+  // host-side copies of the symmetry description
+  std::vector<int> block_dim, block_off, block_modes, sym_block_id;
+};
+
+int ensure_split(Ctx* c, int prec);
+
+}  // namespace tdscha

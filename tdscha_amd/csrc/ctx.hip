@@ -1,0 +1,358 @@
+// Context lifetime, uploads and the perturbed-average C-ABI entry points.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "common.hpp"
+#include "../../include/tdscha_amd.h"
+
+namespace tdscha {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+const char* last_error() { return g_err.c_str(); }
+
+__global__ void expand_images(const double*, const double*, const double*, const int*, const int*,
+                              const int*, const int*, const long*, const int*, const double*, int,
+                              int, int, long, double*, double*, double*);
+int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s);
+int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s);
+
+// pad R1 (n) -> (np) and Y1 (n,n) -> symmetric (np,np); zero outside
+__global__ void pad_inputs(const double* __restrict__ R1, const double* __restrict__ Y1, int n,
+                           int np, double* __restrict__ R1p, double* __restrict__ Y1p) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)np * np) return;
+  const int a = (int)(e / np), b = (int)(e % np);
+  double v = 0.0;
+  if (a < n && b < n) {
+    const double* y = Y1 + (long)chain * n * n;
+    v = 0.5 * (y[(long)a * n + b] + y[(long)b * n + a]);
+  }
+  Y1p[(long)chain * np * np + e] = v;
+  if (e < np) R1p[(long)chain * np + e] = (e < n) ? R1[(long)chain * n + e] : 0.0;
+}
+
+__global__ void symmetrize_out(const double* __restrict__ zsum_all, const double* __restrict__ fsum,
+                               const double* __restrict__ fy, int n, int np, double norm,
+                               double* __restrict__ f_all, double* __restrict__ d2v_all) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) f_all[(long)chain * n + e] = fsum[(long)chain * np + e];
+  if (e >= (long)n * n) return;
+  const int mu = (int)(e / n), nu = (int)(e % n);
+  const double* z = zsum_all + (long)chain * np * np;
+  d2v_all[(long)chain * n * n + e] =
+      norm * (fy[mu] * z[(long)mu * np + nu] + fy[nu] * z[(long)nu * np + mu]);
+}
+
+static double bose(double w, double T) { return T > 0 ? 1.0 / (std::exp(w * RY_TO_K / T) - 1.0) : 0.0; }
+
+static int choose_ksplit(int ntile, long Mp) {
+  // aim at ~6 workgroups per CU for the outer GEMM, multiple of 8 (one per XCD)
+  const int target = 1536;
+  int ks = (target + ntile * ntile - 1) / (ntile * ntile);
+  ks = round_up(ks, 8);
+  ks = std::min(ks, 128);
+  // at least 256 images per split
+  while (ks > 8 && (long)ks * 256 > Mp) ks -= 8;
+  return std::max(ks, 8);
+}
+
+// run the perturbed averages for nc chains whose padded inputs are in c->R1/c->Y1;
+// outputs: unpadded f/d2v at the given device pointers
+int run_pa(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2v_dev, hipStream_t s) {
+  int rc;
+  if (prec == PREC_FP64) {
+    rc = launch_pa_fp64(c, nc, apply_d4, s);
+  } else {
+    rc = launch_pa_split(c, nc, apply_d4, prec, s);
+  }
+  if (rc) return rc;
+  const double norm = 1.0 / (c->n_eff * c->n_syms);
+  dim3 g((unsigned)(((long)c->n * c->n + 255) / 256), nc);
+  hipLaunchKernelGGL(symmetrize_out, g, dim3(256), 0, s, c->d2v_out, c->f_out, c->fy, c->n, c->np,
+                     norm, f_dev, d2v_dev);
+  TDS_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace tdscha
+
+using namespace tdscha;
+
+extern "C" {
+
+const char* tdscha_last_error(void) { return tdscha::last_error(); }
+
+int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double* X,
+                      const double* Y, const double* w, const double* rho, double T, int n_syms,
+                      int n_blocks, const int* block_dim, const int* block_modes,
+                      const double* const* block_syms, const int* sym_block_id, int flags) {
+  (void)flags;
+  TDS_REQUIRE(out != nullptr, 2, "tdscha_ctx_create: out is NULL");
+  *out = nullptr;
+  TDS_REQUIRE(n_modes > 0 && n_configs > 0 && n_syms > 0 && n_blocks > 0, 2,
+              "tdscha_ctx_create: sizes must be positive");
+  TDS_REQUIRE(X && Y && w && rho && block_dim && block_modes && block_syms && sym_block_id, 2,
+              "tdscha_ctx_create: NULL array argument");
+  TDS_REQUIRE(n_modes <= 4096, 2, "tdscha_ctx_create: n_modes above 4096 is not supported");
+  // validate the block description on the host
+  std::vector<int> pos(n_modes, -1);
+  long tot = 0;
+  for (int b = 0; b < n_blocks; ++b) {
+    TDS_REQUIRE(block_dim[b] > 0 && block_dim[b] <= n_modes, 2, "tdscha_ctx_create: bad block_dim");
+    TDS_REQUIRE(block_syms[b] != nullptr, 2, "tdscha_ctx_create: NULL symmetry block");
+    tot += block_dim[b];
+  }
+  TDS_REQUIRE(tot == n_modes, 2, "tdscha_ctx_create: blocks must partition the modes");
+  {
+    long off = 0;
+    for (int b = 0; b < n_blocks; ++b) {
+      for (int k = 0; k < block_dim[b]; ++k) {
+        const int m = block_modes[off + k];
+        TDS_REQUIRE(m >= 0 && m < n_modes && pos[m] < 0, 2,
+                    "tdscha_ctx_create: block_modes is not a permutation of the modes");
+        TDS_REQUIRE(sym_block_id[m] == b, 2, "tdscha_ctx_create: sym_block_id disagrees with blocks");
+        pos[m] = k;
+      }
+      off += block_dim[b];
+    }
+  }
+  for (int m = 0; m < n_modes; ++m) TDS_REQUIRE(w[m] > 0, 2, "tdscha_ctx_create: frequencies must be positive");
+
+  Ctx* c = new Ctx();
+  c->n = n_modes;
+  c->np = round_up(n_modes, kTile);
+  c->N = n_configs;
+  c->n_syms = n_syms;
+  c->M = (long)n_configs * n_syms;
+  c->Mp = round_up_l(c->M, 64);
+  c->T = T;
+  double ne = 0;
+  for (int i = 0; i < n_configs; ++i) ne += rho[i];
+  c->n_eff = ne;
+  TDS_REQUIRE(ne != 0.0, 2, "tdscha_ctx_create: sum of weights is zero");
+  const int np = c->np;
+  const long Mp = c->Mp;
+  c->ksplit = choose_ksplit(np / kTile, Mp);
+
+  auto fail = [&](void) { tdscha_ctx_destroy(reinterpret_cast<tdscha_ctx*>(c)); return -1; };
+#define ALLOC(ptr, bytes)                                                 \
+  if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess) {                 \
+    set_error(std::string("hipMalloc failed for ") + #ptr);               \
+    return fail();                                                        \
+  }
+  if (hipGetDevice(&c->device) != hipSuccess) { set_error("no HIP device"); delete c; return -1; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed"); delete c; return -1;
+  }
+  ALLOC(c->U, sizeof(double) * Mp * np);
+  ALLOC(c->F, sizeof(double) * Mp * np);
+  ALLOC(c->rho_img, sizeof(double) * Mp);
+  ALLOC(c->fy, sizeof(double) * np);
+  ALLOC(c->fpsi, sizeof(double) * np);
+  ALLOC(c->w, sizeof(double) * np);
+  ALLOC(c->R1, sizeof(double) * kMaxChains * np);
+  ALLOC(c->Y1, sizeof(double) * kMaxChains * (long)np * np);
+  ALLOC(c->coef, sizeof(double) * kMaxChains * Mp * 4);
+  ALLOC(c->zpart, sizeof(double) * kMaxChains * (long)c->ksplit * np * np);
+  ALLOC(c->fpart, sizeof(double) * kMaxChains * (long)c->ksplit * np);
+  ALLOC(c->f_out, sizeof(double) * kMaxChains * np);
+  ALLOC(c->d2v_out, sizeof(double) * kMaxChains * (long)np * np);
+  ALLOC(c->stage_in, sizeof(double) * kMaxChains * ((long)n_modes + (long)n_modes * n_modes));
+  ALLOC(c->stage_out, sizeof(double) * kMaxChains * ((long)n_modes + (long)n_modes * n_modes));
+
+  // thermal factors (padded modes get zero)
+  std::vector<double> hfy(np, 0.0), hfp(np, 0.0), hw(np, 0.0);
+  for (int m = 0; m < n_modes; ++m) {
+    const double nb = bose(w[m], T);
+    hfy[m] = 2.0 * w[m] / (1.0 + 2.0 * nb);
+    hfp[m] = (1.0 + 2.0 * nb) / (2.0 * w[m]);
+    hw[m] = w[m];
+  }
+  if (hipMemcpy(c->fy, hfy.data(), sizeof(double) * np, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->fpsi, hfp.data(), sizeof(double) * np, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->w, hw.data(), sizeof(double) * np, hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("upload of thermal factors failed");
+    return fail();
+  }
+
+  // symmetry tables
+  std::vector<int> mode_block(n_modes), mode_pos(n_modes), bmoff(n_blocks);
+  std::vector<long> bsoff(n_blocks);
+  long soff = 0;
+  int moff = 0;
+  for (int b = 0; b < n_blocks; ++b) {
+    bmoff[b] = moff;
+    bsoff[b] = soff;
+    moff += block_dim[b];
+    soff += (long)n_syms * block_dim[b] * block_dim[b];
+  }
+  for (int m = 0; m < n_modes; ++m) {
+    mode_block[m] = sym_block_id[m];
+    mode_pos[m] = pos[m];
+  }
+  std::vector<double> hsyms(soff);
+  for (int b = 0; b < n_blocks; ++b)
+    std::memcpy(hsyms.data() + bsoff[b], block_syms[b],
+                sizeof(double) * (size_t)n_syms * block_dim[b] * block_dim[b]);
+  c->block_dim.assign(block_dim, block_dim + n_blocks);
+  c->block_modes.assign(block_modes, block_modes + n_modes);
+  c->sym_block_id.assign(sym_block_id, sym_block_id + n_modes);
+
+  int *d_mb = nullptr, *d_mp = nullptr, *d_bd = nullptr, *d_bmo = nullptr, *d_bm = nullptr;
+  long* d_bso = nullptr;
+  double *d_syms = nullptr, *d_X = nullptr, *d_Y = nullptr, *d_rho = nullptr;
+  bool ok = hipMalloc((void**)&d_mb, sizeof(int) * n_modes) == hipSuccess &&
+            hipMalloc((void**)&d_mp, sizeof(int) * n_modes) == hipSuccess &&
+            hipMalloc((void**)&d_bd, sizeof(int) * n_blocks) == hipSuccess &&
+            hipMalloc((void**)&d_bmo, sizeof(int) * n_blocks) == hipSuccess &&
+            hipMalloc((void**)&d_bm, sizeof(int) * n_modes) == hipSuccess &&
+            hipMalloc((void**)&d_bso, sizeof(long) * n_blocks) == hipSuccess &&
+            hipMalloc((void**)&d_syms, sizeof(double) * soff) == hipSuccess &&
+            hipMalloc((void**)&d_X, sizeof(double) * (size_t)n_configs * n_modes) == hipSuccess &&
+            hipMalloc((void**)&d_Y, sizeof(double) * (size_t)n_configs * n_modes) == hipSuccess &&
+            hipMalloc((void**)&d_rho, sizeof(double) * n_configs) == hipSuccess;
+  ok = ok && hipMemcpy(d_mb, mode_block.data(), sizeof(int) * n_modes, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(d_mp, mode_pos.data(), sizeof(int) * n_modes, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(d_bd, block_dim, sizeof(int) * n_blocks, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(d_bmo, bmoff.data(), sizeof(int) * n_blocks, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(d_bm, block_modes, sizeof(int) * n_modes, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(d_bso, bsoff.data(), sizeof(long) * n_blocks, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(d_syms, hsyms.data(), sizeof(double) * soff, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(d_X, X, sizeof(double) * (size_t)n_configs * n_modes, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(d_Y, Y, sizeof(double) * (size_t)n_configs * n_modes, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(d_rho, rho, sizeof(double) * n_configs, hipMemcpyHostToDevice) == hipSuccess;
+  if (ok) {
+    const int tpb = 128;
+    dim3 g((unsigned)Mp, (np + tpb - 1) / tpb);
+    hipLaunchKernelGGL(expand_images, g, dim3(tpb), 0, c->stream, d_X, d_Y, d_rho, d_mb, d_mp, d_bd,
+                       d_bmo, d_bso, d_bm, d_syms, n_modes, np, n_syms, c->M, c->U, c->F,
+                       c->rho_img);
+    ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(c->stream) == hipSuccess;
+  }
+  hipFree(d_mb); hipFree(d_mp); hipFree(d_bd); hipFree(d_bmo); hipFree(d_bm); hipFree(d_bso);
+  hipFree(d_syms); hipFree(d_X); hipFree(d_Y); hipFree(d_rho);
+  if (!ok) {
+    set_error("tdscha_ctx_create: upload / image expansion failed");
+    return fail();
+  }
+#undef ALLOC
+  *out = reinterpret_cast<tdscha_ctx*>(c);
+  return 0;
+}
+
+void tdscha_ctx_destroy(tdscha_ctx* ctx) {
+  if (!ctx) return;
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->U, c->F, c->rho_img, c->fy, c->fpsi, c->w, c->R1, c->Y1, c->coef, c->zpart,
+                  c->fpart, c->f_out, c->d2v_out, c->Y1h, c->Y1l, c->ysc, c->bsc, c->cmax,
+                  c->pair_ca, c->pair_cb, c->pair_sm, c->pair_sp, c->pair_harm_a, c->pair_harm_b,
+                  c->pair_row, c->pair_col, c->std_coef, c->yw, c->reaw, c->pols, c->inv_perms,
+                  c->tproj, c->cart_tmp, c->cart_tmp2, c->mode_tmp, c->psi_dev, c->psi_out_dev,
+                  c->stage_in, c->stage_out};
+  for (void* p : ptrs)
+    if (p) hipFree(p);
+  for (auto& s : c->split) {
+    void* sp[] = {s.Uh, s.Ul, s.Fh, s.Fl, s.su, s.sf, s.umax, s.fmax};
+    for (void* p : sp)
+      if (p) hipFree(p);
+  }
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int tdscha_ctx_info(const tdscha_ctx* ctx, int* n_modes, int* n_modes_padded, long* n_images,
+                    long* n_images_padded, int* ksplit) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_ctx_info: NULL context");
+  const Ctx* c = reinterpret_cast<const Ctx*>(ctx);
+  if (n_modes) *n_modes = c->n;
+  if (n_modes_padded) *n_modes_padded = c->np;
+  if (n_images) *n_images = c->M;
+  if (n_images_padded) *n_images_padded = c->Mp;
+  if (ksplit) *ksplit = c->ksplit;
+  return 0;
+}
+
+int tdscha_perturb_averages_dev(tdscha_ctx* ctx, int n_chains, const double* R1_dev,
+                                const double* Y1_dev, int apply_d4, int precision, double* f_dev,
+                                double* d2v_dev, void* stream) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_perturb_averages_dev: NULL context");
+  TDS_REQUIRE(n_chains >= 1 && n_chains <= kMaxChains, 2,
+              "tdscha_perturb_averages_dev: n_chains must be in [1, 8]");
+  TDS_REQUIRE(apply_d4 == 0 || apply_d4 == 1, 2, "the apply_d4 value passed must be either 0 or 1");
+  TDS_REQUIRE(precision >= 0 && precision <= 2, 2, "precision must be 0 (FP64), 1 (FP16x2) or 2 (BF16x2)");
+  TDS_REQUIRE(R1_dev && Y1_dev && f_dev && d2v_dev, 2, "tdscha_perturb_averages_dev: NULL pointer");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : c->stream;
+  const long nn = (long)c->np * c->np;
+  dim3 g((unsigned)((nn + 255) / 256), n_chains);
+  hipLaunchKernelGGL(pad_inputs, g, dim3(256), 0, s, R1_dev, Y1_dev, c->n, c->np, c->R1, c->Y1);
+  TDS_CHECK_HIP(hipGetLastError());
+  return run_pa(c, n_chains, apply_d4, precision, f_dev, d2v_dev, s);
+}
+
+int tdscha_perturb_averages_batched(tdscha_ctx* ctx, int n_chains, const double* R1,
+                                    const double* Y1, int apply_d4, int precision, double* f_out,
+                                    double* d2v_out) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_perturb_averages_batched: NULL context");
+  TDS_REQUIRE(R1 && Y1 && f_out && d2v_out, 2, "tdscha_perturb_averages_batched: NULL pointer");
+  TDS_REQUIRE(n_chains >= 1 && n_chains <= kMaxChains, 2, "n_chains must be in [1, 8]");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  const size_t n = c->n, nn = n * n;
+  double* dR = c->stage_in;
+  double* dY = c->stage_in + (size_t)n_chains * n;
+  double* df = c->stage_out;
+  double* dd = c->stage_out + (size_t)n_chains * n;
+  TDS_CHECK_HIP(hipMemcpyAsync(dR, R1, sizeof(double) * n_chains * n, hipMemcpyHostToDevice, c->stream));
+  TDS_CHECK_HIP(hipMemcpyAsync(dY, Y1, sizeof(double) * n_chains * nn, hipMemcpyHostToDevice, c->stream));
+  int rc = tdscha_perturb_averages_dev(ctx, n_chains, dR, dY, apply_d4, precision, df, dd, c->stream);
+  if (rc) return rc;
+  TDS_CHECK_HIP(hipMemcpyAsync(f_out, df, sizeof(double) * n_chains * n, hipMemcpyDeviceToHost, c->stream));
+  TDS_CHECK_HIP(hipMemcpyAsync(d2v_out, dd, sizeof(double) * n_chains * nn, hipMemcpyDeviceToHost, c->stream));
+  TDS_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int tdscha_perturb_averages(tdscha_ctx* ctx, const double* R1, const double* Y1, int apply_d4,
+                            double* f_out, double* d2v_out) {
+  return tdscha_perturb_averages_batched(ctx, 1, R1, Y1, apply_d4, PREC_FP64, f_out, d2v_out);
+}
+
+// ------------------------------------------------------------ device buffers
+int tdscha_device_alloc(void** out, size_t bytes) {
+  TDS_REQUIRE(out != nullptr, 2, "tdscha_device_alloc: NULL out");
+  *out = nullptr;
+  TDS_CHECK_HIP(hipMalloc(out, bytes ? bytes : 1));
+  return 0;
+}
+
+int tdscha_device_free(void* ptr) {
+  if (ptr) TDS_CHECK_HIP(hipFree(ptr));
+  return 0;
+}
+
+int tdscha_memcpy(void* dst, const void* src, size_t bytes, int kind) {
+  TDS_REQUIRE(kind >= 0 && kind <= 3, 2, "tdscha_memcpy: kind must be 0..3 (H2H, H2D, D2H, D2D)");
+  TDS_CHECK_HIP(hipMemcpy(dst, src, bytes, static_cast<hipMemcpyKind>(kind)));
+  return 0;
+}
+
+int tdscha_device_count(int* count) {
+  TDS_REQUIRE(count != nullptr, 2, "tdscha_device_count: NULL out");
+  *count = 0;
+  if (hipGetDeviceCount(count) != hipSuccess) *count = 0;
+  return 0;
+}
+
+int tdscha_synchronize(tdscha_ctx* ctx) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_synchronize: NULL context");
+  TDS_CHECK_HIP(hipStreamSynchronize(reinterpret_cast<Ctx*>(ctx)->stream));
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,327 @@
+// FP64 ensemble-averaged D3/D4 application on MFMA (v_mfma_f64_16x16x4f64).
+//
+// The per-image algebra of CModules/LanczosFunctions.c:2082-2754 is regrouped
+// into three passes over the device-resident symmetric image ensemble
+// (rows = images b = i*N_sym + j, columns = modes):
+//
+//   weights_f64   V = U . Y1 (tiled GEMM, K = modes) with a fused row-dot
+//                 epilogue: s1 = u.Y1.u, s2 = u.Y1.g, s3 = a.R1, s4 = f.R1
+//                 -> per image c1, c2 (d2v weights) and alpha, beta (force
+//                 weights).
+//   outer_f64     Z = U^T B with B = c1 F + (c2/2) fY U  (split-K GEMM over
+//                 the images, LDS-staged image tiles), and on the diagonal
+//                 tiles the force average sum alpha F + beta fY U.
+//   reduce        d2v = fY Z + (fY Z)^T, f = sum of partials, scaled by
+//                 1/(N_eff N_sym); fixed summation order (deterministic).
+//
+// MFMA f64 16x16x4 operand layout (one double per lane):
+//   A[i][k]: i = lane % 16, k = lane / 16;  B[k][j]: j = lane % 16, k = lane / 16
+//   D[i][j]: j = lane % 16, i = 4 * r + lane / 16  (r = register 0..3; note the
+//   row interleave, unlike the FP32-output MFMAs whose row is 4 * (lane / 16) + r)
+#include "common.hpp"
+
+namespace tdscha {
+
+// ---------------------------------------------------------------------------
+// image expansion: U[b, m] = sum_k S_{blk(m)}[j, pos(m), k] X[i, modes_blk[k]]
+// ---------------------------------------------------------------------------
+__global__ void expand_images(const double* __restrict__ X, const double* __restrict__ Y,
+                              const double* __restrict__ rho, const int* __restrict__ mode_block,
+                              const int* __restrict__ mode_pos, const int* __restrict__ block_dim,
+                              const int* __restrict__ block_moff, const long* __restrict__ block_soff,
+                              const int* __restrict__ block_modes, const double* __restrict__ syms,
+                              int n, int np, int n_syms, long M, double* __restrict__ U,
+                              double* __restrict__ F, double* __restrict__ rho_img) {
+  const long row = blockIdx.x;
+  const int m = threadIdx.x + blockIdx.y * blockDim.x;
+  if (m >= np) return;
+  double su = 0.0, sf = 0.0;
+  if (row < M && m < n) {
+    const long i = row / n_syms;
+    const int j = (int)(row % n_syms);
+    const int b = mode_block[m];
+    const int d = block_dim[b];
+    const double* S = syms + block_soff[b] + ((long)j * d + mode_pos[m]) * d;
+    const int* modes = block_modes + block_moff[b];
+    const double* x = X + i * n;
+    const double* y = Y + i * n;
+    for (int k = 0; k < d; ++k) {
+      const double s = S[k];
+      su += s * x[modes[k]];
+      sf += s * y[modes[k]];
+    }
+  }
+  U[row * np + m] = su;
+  F[row * np + m] = sf;
+  if (m == 0) rho_img[row] = (row < M) ? rho[row / n_syms] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// weights: one workgroup = 64 image rows; column passes of 128; K = np
+// ---------------------------------------------------------------------------
+constexpr int WA_KC = 16;
+
+__global__ __launch_bounds__(256) void weights_f64(
+    const double* __restrict__ U, const double* __restrict__ F,
+    const double* __restrict__ Y1all, const double* __restrict__ R1all,
+    const double* __restrict__ fy, const double* __restrict__ fpsi,
+    const double* __restrict__ rho_img, int np, long Mp, int apply_d4,
+    double* __restrict__ coef_all) {
+  const int chain = blockIdx.y;
+  const double* Y1 = Y1all + (long)chain * np * np;
+  const double* R1 = R1all + (long)chain * np;
+  double* coef = coef_all + (long)chain * Mp * 4;
+  const long r0 = (long)blockIdx.x * 64;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int lr = lane & 15;   // lane within a 16-group
+  const int lg = lane >> 4;   // 16-group index
+
+  __shared__ double sU[64][WA_KC + 1];
+  __shared__ double sY[WA_KC][kTile + 2];
+  __shared__ double red[4][64][4];
+  __shared__ double rowsum[64][4];
+
+  rowsum[tid >> 2][tid & 3] = 0.0;
+
+  for (int c0 = 0; c0 < np; c0 += kTile) {
+    v4d acc[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+
+    for (int k0 = 0; k0 < np; k0 += WA_KC) {
+      __syncthreads();
+      // stage U[r0:r0+64, k0:k0+16]  (1024 values, 4 per thread)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int idx = tid + 256 * e;
+        const int rr = idx >> 4, kk = idx & 15;
+        sU[rr][kk] = U[(r0 + rr) * np + k0 + kk];
+      }
+      // stage Y1[k0:k0+16, c0:c0+128]  (2048 values, 8 per thread)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int idx = tid + 256 * e;
+        const int kk = idx >> 7, cc = idx & 127;
+        sY[kk][cc] = Y1[(long)(k0 + kk) * np + c0 + cc];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < WA_KC; kk += 4) {
+        double a[4], b[2];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb) a[rb] = sU[16 * rb + lr][kk + lg];
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) b[cb] = sY[kk + lg][32 * wave + 16 * cb + lr];
+#pragma unroll
+        for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+          for (int cb = 0; cb < 2; ++cb)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
+      }
+    }
+
+    // fused epilogue: partial row dots over this wave's 32 columns
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      double p[4][4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p[r][q] = 0.0;
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
+        const int col = c0 + 32 * wave + 16 * cb + lr;
+        const double fyc = fy[col], fpc = fpsi[col], r1c = R1[col];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const long row = r0 + 16 * rb + 4 * r + lg;
+          const double u = U[row * np + col];
+          const double f = F[row * np + col];
+          const double v = acc[rb][cb][r];
+          p[r][0] += v * u;
+          p[r][1] += v * f * fpc;
+          p[r][2] += u * fyc * r1c;
+          p[r][3] += f * r1c;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          double v = p[r][q];
+          v += __shfl_xor(v, 1);
+          v += __shfl_xor(v, 2);
+          v += __shfl_xor(v, 4);
+          v += __shfl_xor(v, 8);
+          p[r][q] = v;
+        }
+      if (lr == 0) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) red[wave][16 * rb + 4 * r + lg][q] = p[r][q];
+      }
+    }
+    __syncthreads();
+    {
+      const int rr = tid >> 2, q = tid & 3;
+      rowsum[rr][q] += (red[0][rr][q] + red[1][rr][q]) + (red[2][rr][q] + red[3][rr][q]);
+    }
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const long row = r0 + tid;
+    const double s1 = rowsum[tid][0];  // u.Y1.u
+    const double s2 = rowsum[tid][1];  // u.Y1.g  (Y1 symmetrised on upload)
+    const double s3 = rowsum[tid][2];  // a.R1
+    const double s4 = rowsum[tid][3];  // f.R1
+    const double rho = rho_img[row];
+    double* c = coef + row * 4;
+    c[0] = rho * (-s3 / 3.0 + apply_d4 * s1 / 8.0);
+    c[1] = rho * (-s4 / 3.0 + apply_d4 * s2 / 4.0);
+    c[2] = -rho * s1 / 6.0;
+    c[3] = -rho * s2 / 3.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// outer: split-K Z[mu, nu] = sum_b U[b, mu] B[b, nu] over a range of images
+// ---------------------------------------------------------------------------
+constexpr int WB_KC = 16;
+
+__global__ __launch_bounds__(256) void outer_f64(
+    const double* __restrict__ U, const double* __restrict__ F,
+    const double* __restrict__ coef_all, const double* __restrict__ fy, int np, long Mp,
+    int ksplit, int ntile, long rows_per_split, double* __restrict__ zpart_all,
+    double* __restrict__ fpart_all) {
+  const int id = blockIdx.x;
+  const int xcd = id & 7;
+  const int slot = id >> 3;
+  const int t2 = ntile * ntile;
+  const int ks = xcd + 8 * (slot / t2);
+  const int tile = slot % t2;
+  const int tm = tile / ntile, tn = tile % ntile;
+  const int chain = blockIdx.y;
+  const double* coef = coef_all + (long)chain * Mp * 4;
+  double* zpart = zpart_all + ((long)chain * ksplit + ks) * np * np;
+  double* fpart = fpart_all + ((long)chain * ksplit + ks) * np;
+  const int mu0 = tm * kTile, nu0 = tn * kTile;
+  const bool diag = (tm == tn);
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int lr = lane & 15, lg = lane >> 4;
+  const int wm = wave >> 1, wn = wave & 1;
+
+  __shared__ double sA[WB_KC][kTile + 2];
+  __shared__ double sB[WB_KC][kTile + 2];
+  __shared__ double sF[2][kTile];
+
+  v4d acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  const long kb = (long)ks * rows_per_split;
+  long ke = kb + rows_per_split;
+  if (ke > Mp) ke = Mp;
+  const int col = tid & 127;
+  const int rsub = tid >> 7;  // 0 or 1
+  const double fyn = fy[nu0 + col];
+  double fl = 0.0;
+
+  for (long k0 = kb; k0 < ke; k0 += WB_KC) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < WB_KC / 2; ++e) {
+      const int kk = rsub + 2 * e;
+      const long row = k0 + kk;
+      const double* cf = coef + row * 4;
+      const double c1 = cf[0], c2 = cf[1];
+      const double um = U[row * np + mu0 + col];
+      const double un = diag ? um : U[row * np + nu0 + col];
+      const double fn = F[row * np + nu0 + col];
+      sA[kk][col] = um;
+      sB[kk][col] = c1 * fn + 0.5 * c2 * fyn * un;
+      if (diag) fl += cf[2] * fn + cf[3] * fyn * un;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < WB_KC; kk += 4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = sA[kk + lg][64 * wm + 16 * i + lr];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = sB[kk + lg][64 * wn + 16 * j + lr];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+  }
+  // write the partial tile
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mu = mu0 + 64 * wm + 16 * i + 4 * r + lg;
+        const int nu = nu0 + 64 * wn + 16 * j + lr;
+        zpart[(long)mu * np + nu] = acc[i][j][r];
+      }
+  if (diag) {
+    sF[rsub][col] = fl;
+    __syncthreads();
+    if (tid < kTile) fpart[nu0 + tid] = sF[0][tid] + sF[1][tid];
+  }
+}
+
+// sum the K-split partials (fixed order): zsum[mu, nu], f
+__global__ void reduce_ksplit(const double* __restrict__ zpart_all,
+                              const double* __restrict__ fpart_all, int np, int ksplit,
+                              double* __restrict__ zsum_all, double* __restrict__ f_all,
+                              double norm) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nn = (long)np * np;
+  const double* zp = zpart_all + (long)chain * ksplit * nn;
+  if (e < nn) {
+    double s = 0.0;
+    for (int k = 0; k < ksplit; ++k) s += zp[(long)k * nn + e];
+    zsum_all[(long)chain * nn + e] = s;
+  }
+  if (e < np) {
+    const double* fp = fpart_all + (long)chain * ksplit * np;
+    double s = 0.0;
+    for (int k = 0; k < ksplit; ++k) s += fp[(long)k * np + e];
+    f_all[(long)chain * np + e] = s * norm;
+  }
+}
+
+int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s) {
+  const int ntile = c->np / kTile;
+  dim3 ga((unsigned)(c->Mp / 64), nc);
+  hipLaunchKernelGGL(weights_f64, ga, dim3(256), 0, s, c->U, c->F, c->Y1, c->R1, c->fy, c->fpsi,
+                     c->rho_img, c->np, c->Mp, apply_d4, c->coef);
+  const long rps = round_up_l((c->Mp + c->ksplit - 1) / c->ksplit, WB_KC);
+  dim3 gb((unsigned)(ntile * ntile * c->ksplit), nc);
+  hipLaunchKernelGGL(outer_f64, gb, dim3(256), 0, s, c->U, c->F, c->coef, c->fy, c->np, c->Mp,
+                     c->ksplit, ntile, rps, c->zpart, c->fpart);
+  const long nn = (long)c->np * c->np;
+  const double norm = 1.0 / (c->n_eff * c->n_syms);
+  // zsum -> c->d2v_out (padded), normalised f -> c->f_out
+  dim3 gr((unsigned)((nn + 255) / 256), nc);
+  hipLaunchKernelGGL(reduce_ksplit, gr, dim3(256), 0, s, c->zpart, c->fpart, c->np, c->ksplit,
+                     c->d2v_out, c->f_out, norm);
+  return 0;
+}
+
+}  // namespace tdscha

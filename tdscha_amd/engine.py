@@ -1,0 +1,159 @@
+"""Device-resident ensemble + perturbed averages on the gfx950 library.
+
+``Engine`` owns one ``tdscha_ctx``: the symmetric image ensemble lives in HBM
+from construction on (the reference re-marshals X and Y on every call,
+report/profiling_report.md:84-85).  ``GetPerturbAverageSym`` reproduces the
+reference binding's argument list and in-place semantics
+(CModules/odd_corr_module.c:491-634) on top of a cached Engine.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+FP64, FP16X2, BF16X2 = 0, 1, 2
+_PREC = {"fp64": FP64, "fp16x2": FP16X2, "bf16x2": BF16X2}
+
+
+def precision_code(p):
+    if isinstance(p, str):
+        return _PREC[p.lower()]
+    return int(p)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Engine:
+    """Ensemble (X, Y, rho), frequencies, temperature and block symmetries."""
+
+    def __init__(self, X, Y, w, rho, T, symmetries, degenerate_space, sym_block_id):
+        L = _lib.lib()
+        self._L = L
+        X = np.ascontiguousarray(X, np.float64)
+        Y = np.ascontiguousarray(Y, np.float64)
+        w = np.ascontiguousarray(w, np.float64)
+        rho = np.ascontiguousarray(rho, np.float64)
+        if X.ndim != 2 or X.shape != Y.shape:
+            raise ValueError("X and Y must be (N_configs, n_modes) arrays of equal shape")
+        N, n = X.shape
+        if len(w) != n:
+            raise ValueError("N_modes from X is %d, while len(w) = %d" % (n, len(w)))
+        if len(rho) != N:
+            raise ValueError("len(rho) must equal the number of configurations")
+        syms = [np.ascontiguousarray(s, np.float64) for s in symmetries]
+        blocks = [np.ascontiguousarray(b, np.int32) for b in degenerate_space]
+        n_syms = syms[0].shape[0]
+        for s, b in zip(syms, blocks):
+            if s.shape != (n_syms, len(b), len(b)):
+                raise ValueError("symmetry block of shape %s does not match its %d modes"
+                                 % (s.shape, len(b)))
+        self._keep = (X, Y, w, rho, syms, blocks)
+        dims = np.array([len(b) for b in blocks], dtype=np.int32)
+        modes = np.ascontiguousarray(np.concatenate(blocks).astype(np.int32))
+        sbi = np.ascontiguousarray(sym_block_id, np.int32)
+        ptrs = (C.c_void_p * len(syms))(*[s.ctypes.data for s in syms])
+        ctx = C.c_void_p()
+        rc = L.tdscha_ctx_create(C.byref(ctx), n, N, _ptr(X), _ptr(Y), _ptr(w), _ptr(rho),
+                                 float(T), n_syms, len(blocks), _ptr(dims), _ptr(modes),
+                                 C.cast(ptrs, C.c_void_p), _ptr(sbi), 0)
+        _lib.check(rc, "tdscha_ctx_create")
+        self.ctx = ctx
+        self.n_modes = n
+        self.n_configs = N
+        self.n_syms = n_syms
+        self.T = float(T)
+        ni = [C.c_int(), C.c_int(), C.c_long(), C.c_long(), C.c_int()]
+        _lib.check(L.tdscha_ctx_info(ctx, C.byref(ni[0]), C.byref(ni[1]), C.byref(ni[2]),
+                                     C.byref(ni[3]), C.byref(ni[4])), "tdscha_ctx_info")
+        self.n_modes_padded = ni[1].value
+        self.n_images = ni[2].value
+        self.n_images_padded = ni[3].value
+        self.ksplit = ni[4].value
+
+    # -- lifetime --------------------------------------------------------
+    def close(self):
+        if getattr(self, "ctx", None):
+            self._L.tdscha_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _lib.check(self._L.tdscha_synchronize(self.ctx), "tdscha_synchronize")
+
+    # -- perturbed averages ------------------------------------------------
+    def perturb_averages(self, R1, Y1, apply_d4=1):
+        """Host FP64 call: returns (f_pert_av[n], d2v_pert_av[n, n])."""
+        n = self.n_modes
+        R1 = np.ascontiguousarray(R1, np.float64)
+        Y1 = np.ascontiguousarray(Y1, np.float64)
+        if R1.shape != (n,) or Y1.shape != (n, n):
+            raise ValueError("R1 must be (n,) and Y1 (n, n)")
+        f = np.zeros(n)
+        d2v = np.zeros((n, n))
+        rc = self._L.tdscha_perturb_averages(self.ctx, _ptr(R1), _ptr(Y1), int(apply_d4),
+                                             _ptr(f), _ptr(d2v))
+        _lib.check(rc, "tdscha_perturb_averages")
+        return f, d2v
+
+    def perturb_averages_batched(self, R1, Y1, apply_d4=1, precision=FP64):
+        """Host call for nc perturbations at once: R1 (nc, n), Y1 (nc, n, n)."""
+        n = self.n_modes
+        R1 = np.ascontiguousarray(R1, np.float64)
+        Y1 = np.ascontiguousarray(Y1, np.float64)
+        nc = R1.shape[0]
+        if R1.shape != (nc, n) or Y1.shape != (nc, n, n):
+            raise ValueError("R1 must be (nc, n) and Y1 (nc, n, n)")
+        f = np.zeros((nc, n))
+        d2v = np.zeros((nc, n, n))
+        rc = self._L.tdscha_perturb_averages_batched(self.ctx, nc, _ptr(R1), _ptr(Y1),
+                                                     int(apply_d4), precision_code(precision),
+                                                     _ptr(f), _ptr(d2v))
+        _lib.check(rc, "tdscha_perturb_averages_batched")
+        return f, d2v
+
+    def perturb_averages_dev(self, nc, R1_dev, Y1_dev, f_dev, d2v_dev, apply_d4=1,
+                             precision=FP64):
+        """Device call on DeviceBuffer objects (asynchronous on the context stream)."""
+        rc = self._L.tdscha_perturb_averages_dev(
+            self.ctx, int(nc), R1_dev.ptr, Y1_dev.ptr, int(apply_d4), precision_code(precision),
+            f_dev.ptr, d2v_dev.ptr, None)
+        _lib.check(rc, "tdscha_perturb_averages_dev")
+
+
+_engine_cache = {}
+
+
+def GetPerturbAverageSym(X, Y, w, rho, R1, Y1, T, apply_d4, n_syms, symmetries, N_degeneracy,
+                         degenerate_space, sym_block_id, f_pert_av, d2v_pert_av):
+    """Same arguments and in-place outputs as sscha_HP_odd.GetPerturbAverageSym.
+
+    (CModules/odd_corr_module.c:491-634.)  The ensemble is uploaded once and
+    cached per (X, Y) buffer; later calls only move R1/Y1 and the outputs.
+    Invalid arguments raise ValueError (the reference calls exit()).
+    """
+    if apply_d4 not in (0, 1):
+        raise ValueError("the apply_D4 value passed must be either 0 or 1 (got %r)" % (apply_d4,))
+    if np.shape(X)[1] != len(w):
+        raise ValueError("N_modes from X is %d, while len(w) = %d" % (np.shape(X)[1], len(w)))
+    if int(n_syms) != np.shape(symmetries[0])[0]:
+        raise ValueError("n_syms does not match the symmetry blocks")
+    key = (X.ctypes.data, Y.ctypes.data, np.shape(X), float(T), int(n_syms),
+           rho.ctypes.data, w.ctypes.data)
+    eng = _engine_cache.get(key)
+    if eng is None:
+        eng = Engine(X, Y, w, rho, T, symmetries, degenerate_space, sym_block_id)
+        _engine_cache.clear()
+        _engine_cache[key] = eng
+    f, d2v = eng.perturb_averages(R1, Y1, apply_d4)
+    f_pert_av[:] = f
+    d2v_pert_av[:, :] = d2v

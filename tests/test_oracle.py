@@ -1,0 +1,197 @@
+"""Pins of the CPU oracle against the reference's own golden data and the
+mathematics (CPU only)."""
+import numpy as np
+import pytest
+
+from oracle import c_oracle as CO
+from oracle import lanczos_oracle as O
+from tests.systems import load_snte, random_block_system, random_perturbation
+
+
+@pytest.fixture(scope="module")
+def snte():
+    return load_snte()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_symmetry_builder_reproduces_reference_blocks(snte):
+    ms = snte["syms_full"]
+    assert ms.n_syms == 384
+    assert list(ms.N_degeneracy) == list(snte["ref_ndegs"])
+    assert np.array_equal(ms.sym_block_id, snte["ref_blockid"])
+    assert np.array_equal(np.concatenate(ms.degenerate_space), snte["ref_blocks_concat"])
+    assert ms.leakage < 1e-12
+    # the mode-space matrices form a group of orthogonal matrices
+    S = ms.dense()
+    assert np.max(np.abs(np.einsum("jab,jcb->jac", S, S) - np.eye(45))) < 1e-12
+    assert snte["syms_gamma"].n_syms == 48
+
+
+@pytest.mark.parametrize("apply_d4", [0, 1])
+def test_oracle_matches_reference_c_kernel(snte, apply_d4):
+    """numpy and C restatements vs LanczosFunctions.c compiled from its sources."""
+    ms = snte["syms_full"]
+    R1, Y1 = random_perturbation(45, seed=3)
+    args = (snte["X"], snte["Y"], snte["w"], snte["rho"], R1, Y1, 250.0, apply_d4)
+    ref = CO.ref_perturb_averages(*args, ms.symmetries, ms.degenerate_space, ms.sym_block_id)
+    if ref is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    f0, d0 = O.perturb_averages(*args, ms.dense())
+    f1, d1 = CO.oracle_perturb_averages(*args, ms.symmetries, ms.degenerate_space,
+                                        ms.sym_block_id, n_threads=2)
+    assert rel(f0, ref[0]) < 1e-13 and rel(d0, ref[1]) < 1e-13
+    assert rel(f1, ref[0]) < 1e-13 and rel(d1, ref[1]) < 1e-13
+
+
+def _snte_system(snte, use_wigner=False, gamma=False):
+    ms = snte["syms_gamma"] if gamma else snte["syms_full"]
+    kw = {}
+    if gamma:
+        kw = dict(pols=snte["pols"], trans_projector=ms.trans_projector,
+                  inv_perms=ms.trans_cart_perms)
+    return O.OracleSystem(snte["X"], snte["Y"], snte["w"], snte["rho"], float(snte["T"]),
+                          ms.dense(), use_wigner=use_wigner, **kw)
+
+
+def test_oracle_reproduces_golden_L_applications(snte):
+    """LANCZOS_STEP10.npz: L q_k and L^T p_k implied by the stored recurrence."""
+    sysO = _snte_system(snte)
+    a, b, c, sn = snte["a_coeffs"], snte["b_coeffs"], snte["c_coeffs"], snte["s_norm"]
+    Q, P = snte["basis_Q"], snte["basis_P"]
+    for k in range(10):
+        Lq = a[k] * Q[k] + b[k] * Q[k + 1] + (c[k - 1] * Q[k - 1] if k > 0 else 0)
+        assert rel(sysO.apply_full_L(Q[k]), Lq) < 1e-9
+        pn = sn[k] / (c[k - 1] if k > 0 else 1.0)
+        LtP = a[k] * P[k] + (sn[k + 1] / pn) * P[k + 1]
+        if k > 0:
+            opn = sn[k - 1] if k == 1 else sn[k - 1] / c[k - 2]
+            LtP = LtP + b[k - 1] * (opn / pn) * P[k - 1]
+        assert rel(sysO.apply_full_L(P[k], transpose=True), LtP) < 1e-12
+
+
+def test_oracle_reproduces_golden_coefficients(snte):
+    """10-step biconjugate run from prepare_mode(10) (test_julia_lf.py:24-45)."""
+    sysO = _snte_system(snte)
+    psi0 = np.zeros(O.len_psi(45))
+    psi0[10] = 1.0
+    st = O.run_FT(lambda v: sysO.apply_full_L(v), lambda v: sysO.apply_full_L(v, transpose=True),
+                  psi0, 10, run_simm=False)
+    a = np.array(st.a)
+    bc = np.array(st.b) * np.array(st.c)
+    assert np.max(np.abs(a / snte["a_coeffs"] - 1)) < 1e-11
+    assert np.max(np.abs(bc / (snte["b_coeffs"] * snte["c_coeffs"]) - 1)) < 1e-8
+    # the first seven b, c individually (later ones sit at a near-breakdown)
+    assert np.max(np.abs(np.array(st.b[:7]) / snte["b_coeffs"][:7] - 1)) < 1e-12
+    assert abs(st.c[4] - 6.59744344e-07) / 6.59744344e-07 < 1e-6
+    assert np.allclose(st.s_norm, snte["s_norm"], rtol=1e-6)
+
+
+def test_gamma_only_equals_full_group(snte):
+    """48 point-group images + translational projector == 384 images, for a
+    Gamma (translation-invariant) Lanczos vector (test_gamma_only.py:59-74)."""
+    full = _snte_system(snte, use_wigner=True)
+    gam = _snte_system(snte, use_wigner=True, gamma=True)
+    # a Gamma perturbation: IR-like rigid sublattice displacement along x
+    nat = 16
+    v = np.zeros(3 * nat)
+    v[0::6] = 1.0
+    v[3::6] = -1.0
+    psi = np.zeros(O.len_psi(45))
+    psi[:45] = np.einsum("a,a,ab->b", np.sqrt(snte["m"]) ** -1, v, snte["pols"])
+    for _ in range(3):
+        lf = full.apply_full_L(psi)
+        lg = gam.apply_full_L(psi)
+        assert rel(lg, lf) < 1e-11
+        psi = lf / np.linalg.norm(lf)
+
+
+@pytest.mark.parametrize("use_wigner", [True, False])
+def test_fast_kernel_matches_dense_explicit_tensor_L(use_wigner):
+    """Brute force: the explicit D3/D4-tensor L (DL:5167-5900) at tiny n."""
+    s = random_block_system(6, 25, 1, [1] * 6, seed=5, T=200.0)
+    s["Y"] *= 1e5
+    sysO = O.OracleSystem(s["X"] * 1e-1, s["Y"], s["w"] * 1e2, s["rho"], s["T"] * 10,
+                          None, use_wigner=use_wigner)
+    L = O.dense_L(sysO)
+    rng = np.random.default_rng(2)
+    for _ in range(3):
+        psi = rng.normal(size=O.len_psi(6))
+        assert rel(sysO.apply_full_L(psi), L @ psi) < 1e-12
+    if not use_wigner:
+        psi = rng.normal(size=O.len_psi(6))
+        # transposed application of the fast code vs dense L^T
+        assert rel(sysO.apply_full_L(psi, transpose=True), L.T @ psi) < 1e-12
+
+
+def test_wigner_L_is_symmetric_in_the_mask_metric():
+    s = random_block_system(8, 40, 3, [2, 2, 1, 3], seed=7)
+    sysO = O.OracleSystem(s["X"], s["Y"], s["w"], s["rho"], s["T"],
+                          O.dense_symmetries(8, s["syms"], s["blocks"]), use_wigner=True)
+    m = O.mask_dot_wigner(8)
+    rng = np.random.default_rng(4)
+    u, v = rng.normal(size=O.len_psi(8)), rng.normal(size=O.len_psi(8))
+    lhs = u.dot(sysO.apply_full_L(v) * m)
+    rhs = sysO.apply_full_L(u).dot(v * m)
+    assert abs(lhs - rhs) < 1e-12 * max(abs(lhs), 1e-300) * 1e3
+    st = O.run_FT(sysO.apply_full_L, None, u, 8, run_simm=True, mask=m)
+    assert np.allclose(st.b, st.c, rtol=1e-10)
+
+
+def test_harmonic_single_mode_chain_terminates():
+    """ignore_v3: L is diagonal, a single R mode gives a_0 = -w^2 and stops."""
+    s = random_block_system(5, 10, 1, [1] * 5, seed=1)
+    sysO = O.OracleSystem(s["X"], s["Y"], s["w"], s["rho"], s["T"], None, use_wigner=True,
+                          ignore_v3=True)
+    psi = np.zeros(O.len_psi(5))
+    psi[2] = 1.0
+    st = O.run_FT(sysO.apply_full_L, None, psi, 5, run_simm=True, mask=O.mask_dot_wigner(5))
+    assert len(st.a) == 1 and abs(st.a[0] + s["w"][2] ** 2) < 1e-18
+
+
+def test_partition_invariance():
+    """Any split of the (config x sym) index range sums to the total
+    (test_mpi_parallel.py:115-123; DL:3141-3164)."""
+    s = random_block_system(12, 30, 4, [3, 3, 2, 4], seed=9)
+    R1, Y1 = random_perturbation(12, seed=2)
+    args = (s["X"], s["Y"], s["w"], s["rho"], R1, Y1, s["T"], 1, s["syms"], s["blocks"], s["sbi"])
+    f, d = CO.oracle_perturb_averages(*args)
+    cuts = [0, 17, 55, 91, 120]
+    fs = np.zeros_like(f)
+    ds = np.zeros_like(d)
+    for lo, hi in zip(cuts[:-1], cuts[1:]):
+        fp, dp = CO.oracle_perturb_averages(*args, b_begin=lo, b_end=hi)
+        fs += fp
+        ds += dp
+    assert rel(fs, f) < 1e-13 and rel(ds, d) < 1e-13
+
+
+def test_continued_fraction_closed_forms():
+    """tests/test_continued_fraction.py:33-95 of the reference, restated."""
+    a, b, c = [3.0, 10.0], [1.0, 1.0], [1.0, 1.0]
+    w = np.array([0.0])
+    gf = O.continued_fraction(a, b, c, w, use_wigner=False, use_terminator=True)
+    gm = (10.0 - np.sqrt(96 + 0j)) / 2
+    assert np.allclose(gf[0], 1.0 / (3.0 - gm), rtol=1e-10)
+    gfw = O.continued_fraction(a, b, c, w, use_wigner=True, use_terminator=True)
+    e = 1.0 / (3.0 - gm)
+    assert np.allclose(gfw[0], -np.real(e) + 1j * np.imag(e), rtol=1e-10)
+    a2 = [1.0, 2.0, 3.0, 4.0, 5.0]
+    b2 = [0.5, 0.6, 0.7, 0.8, 0.9]
+    w2 = np.array([1.0])
+    g0 = O.continued_fraction(a2, b2, b2, w2, False, True, 2, 0.0, 0)
+    g3 = O.continued_fraction(a2, b2, b2, w2, False, True, 2, 0.0, 3)
+    assert np.isfinite(g3[0]) and not np.allclose(g0, g3, rtol=1e-6)
+    n1 = O.continued_fraction(a2, b2, b2, w2, False, False, 1, 0.01, 0)
+    n5 = O.continued_fraction(a2, b2, b2, w2, False, False, 1, 0.01, 5)
+    assert np.allclose(n1, n5, rtol=1e-14)
+    # without terminator the fraction is the resolvent of the tridiagonal matrix
+    a3 = np.array([1.0, 2.0, 4.0])
+    b3 = np.array([0.3, 0.5, 0.0])
+    wz = np.array([0.7])
+    g = O.continued_fraction(a3, b3, b3, wz, False, False, 1, 0.05)
+    Tm = np.diag(a3) + np.diag(b3[:2], 1) + np.diag(b3[:2], -1)
+    z = wz[0] ** 2 - 2j * wz[0] * 0.05
+    assert np.allclose(g[0], np.linalg.inv(Tm - z * np.eye(3))[0, 0], rtol=1e-12)
