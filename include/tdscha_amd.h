@@ -83,6 +83,63 @@ int tdscha_perturb_averages_batched(tdscha_ctx* ctx, int n_chains, const double*
                                     double* d2v_out);
 
 /*
+ * Gamma-only translational projection (Modules/DynamicalLanczos.py:747-791,
+ * 3174-3191): after the point-group image average, f <- P_T f and
+ * d2v <- pols^T [ (1/n_cells) sum_R P_R (pols d2v pols^T) P_R^T ] pols.
+ *   pols [n_cart, n] (host), inv_perms [n_cells, n_cart] Cartesian index maps
+ *   (d_cart[inv_perm[J], inv_perm[K]] is the translated matrix), trans_projector
+ *   [n, n] = mean of pols^T P_R pols.  n_cells = 0 disables the projection.
+ */
+int tdscha_set_translations(tdscha_ctx* ctx, int n_cart, const double* pols, int n_cells,
+                            const int* inv_perms, const double* trans_projector);
+
+/*
+ * L application (Modules/DynamicalLanczos.py:3549-3651) for n_vec (1..8)
+ * vectors of length n + n(n+1): harmonic part (DL:2287-2445) plus the
+ * anharmonic part (DL:2951-3363), then out = (reverse ? -out : out) + shift*psi.
+ * use_wigner selects the Wigner (a', b') or the standard (Upsilon, ReA)
+ * representation; transpose applies L^T in the standard one (ignored in the
+ * Wigner one, as in the reference).  ignore_v3 drops the anharmonic part,
+ * ignore_v4 the D4 term.  precision as for the perturbed averages.
+ */
+int tdscha_apply_L(tdscha_ctx* ctx, int n_vec, const double* psi, double* out, int use_wigner,
+                   int transpose, int ignore_harmonic, int ignore_v3, int ignore_v4, int reverse,
+                   double shift, int precision);
+int tdscha_apply_L_dev(tdscha_ctx* ctx, int n_vec, const double* psi_dev, double* out_dev,
+                       int use_wigner, int transpose, int ignore_harmonic, int ignore_v3,
+                       int ignore_v4, int reverse, double shift, int precision, void* stream);
+
+/*
+ * Device-resident symmetric Lanczos in the Wigner representation
+ * (run_FT with run_simm, Modules/DynamicalLanczos.py:6032-6240) for n_chains
+ * (1..8) starting vectors psi0 [n_chains, n + n(n+1)] (host), batched through
+ * every L application.  Each chain stops at |b| < tol or after caps[c] steps
+ * (caps may be NULL: n_steps for all).  reorth = 1 adds full classical
+ * Gram-Schmidt reorthogonalisation (two passes) against the whole basis.
+ * use_graph = 1 captures one step as a hipGraph and replays it.
+ * Outputs: a_out, b_out [n_chains, n_steps]; n_done [n_chains] = number of a
+ * coefficients; basis_out (optional) [n_chains, n_steps + 1, len];
+ * elapsed_ms (optional) device time of the step loop.
+ */
+int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_steps,
+                       const int* caps, double tol, int reorth, int precision,
+                       int ignore_harmonic, int ignore_v3, int ignore_v4, int use_graph,
+                       double* a_out, double* b_out, int* n_done, double* basis_out,
+                       double* elapsed_ms);
+
+/*
+ * Continued-fraction Green function with the standard terminator
+ * (Modules/DynamicalLanczos.py:5032-5162) on n_w frequencies.  a [n_a],
+ * b [n_b] (n_b >= n_a - 1), c [n_b] or NULL (c = b).  gf_out [n_w, 2] holds
+ * (Re, Im), Wigner sign convention applied, times modulus.
+ */
+int tdscha_continued_fraction(int n_a, const double* a, int n_b, const double* b,
+                              const double* c, int n_w, const double* w, int use_wigner,
+                              int use_terminator, int last_average, double smearing,
+                              int smooth_ramp, int reverse, double shift, double modulus,
+                              double* gf_out);
+
+/*
  * Device memory helpers, so callers need no other GPU runtime for buffer
  * handoff.  kind: 0 host->host, 1 host->device, 2 device->host, 3 device->device.
  */

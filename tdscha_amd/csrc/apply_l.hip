@@ -1,0 +1,415 @@
+// Device-resident L application (Modules/DynamicalLanczos.py:3549-3651).
+//
+//   psi -> (R1, Y1)               prepare_pert   (DL:3009-3053, 2773-2882)
+//   perturbed averages            run_pa         (the MFMA kernels)
+//   gamma-only translations       project_*      (DL:3174-3191)
+//   harmonic + output assembly    assemble_out   (DL:2287-2445, 3282-3363, 3639-3641)
+//
+// psi layout: [R (n) | sector A: packed upper triangle (n(n+1)/2) | sector B],
+// pairs p in row-major upper-triangle order (row r <= col c), w_a = w[c],
+// w_b = w[r] (DL:2339-2346).
+#include <cmath>
+#include <vector>
+
+#include "common.hpp"
+#include "../../include/tdscha_amd.h"
+
+namespace tdscha {
+
+int run_pa(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2v_dev, hipStream_t s);
+
+// coefficient rows of c->std_coef
+enum { SC_X = 0, SC_Y = 1, SC_X1 = 2, SC_Y1 = 3 };
+
+static double bose_h(double w, double T) {
+  return T > 0 ? 1.0 / (std::exp(w / (T / RY_TO_K)) - 1.0) : 0.0;
+}
+
+int setup_pairs(Ctx* c) {
+  if (c->pair_row) return 0;
+  const int n = c->n;
+  const long P = (long)n * (n + 1) / 2;
+  std::vector<int> pr(P), pc(P);
+  std::vector<double> ca(P), cb(P), sm(P), sp(P), ha(P), hb(P), sc(4 * P);
+  std::vector<double> wv(n);
+  TDS_CHECK_HIP(hipMemcpy(wv.data(), c->w, sizeof(double) * n, hipMemcpyDeviceToHost));
+  long p = 0;
+  for (int r = 0; r < n; ++r)
+    for (int col = r; col < n; ++col, ++p) {
+      pr[p] = r;
+      pc[p] = col;
+      const double wa = wv[col], wb = wv[r];
+      const double na = bose_h(wa, c->T), nb = bose_h(wb, c->T);
+      const double X = (1 + 2 * na) * (1 + 2 * nb) / 8;
+      const double w2x = (wa * wb) / X;
+      const double chm = ((wa - wb) * (na - nb)) / (2 * wa * wb);
+      const double chp = ((wa + wb) * (1 + na + nb)) / (2 * wa * wb);
+      sm[p] = std::sqrt(-0.5 * chm);
+      sp[p] = std::sqrt(0.5 * chp);
+      ca[p] = w2x * sm[p];
+      cb[p] = w2x * sp[p];
+      ha[p] = -(wa * wa + wb * wb - 2.0 * wa * wb);
+      hb[p] = -(wa * wa + wb * wb + 2.0 * wa * wb);
+      const double d = (2 * na + 1) * (2 * nb + 1);
+      sc[SC_X * P + p] = -wa * wa - wb * wb - (2 * wa * wb) / d;
+      sc[SC_Y * P + p] = -(8 * wa * wb) / d;
+      sc[SC_X1 * P + p] = -(2 * na * nb + na + nb) * (2 * na * nb + na + nb + 1) * (2 * wa * wb) / d;
+      sc[SC_Y1 * P + p] = -wa * wa - wb * wb + (2 * wa * wb) / d;
+    }
+  std::vector<double> yw(c->np, 0.0), rw(c->np, 0.0);
+  for (int m = 0; m < n; ++m) {
+    const double nm = c->T > 1e-12 ? 1.0 / (std::exp(wv[m] * RY_TO_K / c->T) - 1.0) : 0.0;
+    yw[m] = 2 * wv[m] / (2 * nm + 1);
+    rw[m] = 2 * wv[m] * nm * (nm + 1) / (2 * nm + 1);
+  }
+  auto up = [&](void** dst, const void* src, size_t bytes) -> int {
+    TDS_CHECK_HIP(hipMalloc(dst, bytes));
+    TDS_CHECK_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+  };
+  int rc = 0;
+  rc |= up((void**)&c->pair_row, pr.data(), sizeof(int) * P);
+  rc |= up((void**)&c->pair_col, pc.data(), sizeof(int) * P);
+  rc |= up((void**)&c->pair_ca, ca.data(), sizeof(double) * P);
+  rc |= up((void**)&c->pair_cb, cb.data(), sizeof(double) * P);
+  rc |= up((void**)&c->pair_sm, sm.data(), sizeof(double) * P);
+  rc |= up((void**)&c->pair_sp, sp.data(), sizeof(double) * P);
+  rc |= up((void**)&c->pair_harm_a, ha.data(), sizeof(double) * P);
+  rc |= up((void**)&c->pair_harm_b, hb.data(), sizeof(double) * P);
+  rc |= up((void**)&c->std_coef, sc.data(), sizeof(double) * 4 * P);
+  rc |= up((void**)&c->yw, yw.data(), sizeof(double) * c->np);
+  rc |= up((void**)&c->reaw, rw.data(), sizeof(double) * c->np);
+  if (rc) return -1;
+  const long len = n + 2 * P;
+  TDS_CHECK_HIP(hipMalloc((void**)&c->psi_dev, sizeof(double) * kMaxChains * len));
+  TDS_CHECK_HIP(hipMalloc((void**)&c->psi_out_dev, sizeof(double) * kMaxChains * len));
+  TDS_CHECK_HIP(hipMalloc((void**)&c->f_fin, sizeof(double) * kMaxChains * n));
+  TDS_CHECK_HIP(hipMalloc((void**)&c->d2v_fin, sizeof(double) * kMaxChains * (long)n * n));
+  return 0;
+}
+
+// psi -> padded R1 / symmetric Y1 (one thread per padded matrix element)
+__global__ void prepare_pert(const double* __restrict__ psi_all, long len, int n, int np,
+                             const double* __restrict__ ca, const double* __restrict__ cb,
+                             const double* __restrict__ yw, const double* __restrict__ reaw,
+                             int use_wigner, int transpose, double* __restrict__ R1p,
+                             double* __restrict__ Y1p) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)np * np) return;
+  const double* psi = psi_all + chain * len;
+  const int a = (int)(e / np), b = (int)(e % np);
+  double v = 0.0;
+  if (a < n && b < n) {
+    const int r = a < b ? a : b, col = a < b ? b : a;
+    const long P = (long)n * (n + 1) / 2;
+    const long p = (long)r * n - (long)r * (r - 1) / 2 + (col - r);
+    const double pa = psi[n + p], pb = psi[n + P + p];
+    if (use_wigner) {
+      v = ca[p] * pa - cb[p] * pb;
+    } else if (!transpose) {
+      v = pa;
+    } else {
+      const double h = (r == col) ? 1.0 : 0.5;
+      const double ya = yw[a], yb = yw[b];
+      const double cY = 2.0 * (ya * yb * yb + ya * ya * yb);
+      const double cRA = 2.0 * ya * yb * (reaw[a] + reaw[b]);
+      v = h * pa * cY + h * pb * cRA;
+    }
+  }
+  Y1p[(long)chain * np * np + e] = v;
+  if (e < np) R1p[(long)chain * np + e] = (e < n) ? psi[e] : 0.0;
+}
+
+// final (normalised, unpadded) f and d2v from the K-split sums
+__global__ void finalize_pa(const double* __restrict__ zsum_all, const double* __restrict__ fsum,
+                            const double* __restrict__ fy, int n, int np, double norm,
+                            double* __restrict__ f_all, double* __restrict__ d2v_all) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < n) f_all[(long)chain * n + e] = fsum[(long)chain * np + e];
+  if (e >= (long)n * n) return;
+  const int mu = (int)(e / n), nu = (int)(e % n);
+  const double* z = zsum_all + (long)chain * np * np;
+  d2v_all[(long)chain * n * n + e] =
+      norm * (fy[mu] * z[(long)mu * np + nu] + fy[nu] * z[(long)nu * np + mu]);
+}
+
+// ---------------------------------------------------------------- small GEMM
+// C[b] = op(A[b]) * op(B[b]); row-major; 64x64 tiles, 256 threads, 4x4 per thread
+__global__ __launch_bounds__(256) void gemm_f64(int M, int N, int K, const double* __restrict__ A,
+                                                long lda, long sA, int ta,
+                                                const double* __restrict__ B, long ldb, long sB,
+                                                int tb, double* __restrict__ C, long ldc, long sC) {
+  const int b = blockIdx.z;
+  A += b * sA;
+  B += b * sB;
+  C += b * sC;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  double acc[4][4] = {};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+      const int mm = e / 16, kk = e % 16;
+      const int gm = m0 + mm, gk = k0 + kk;
+      double va = 0.0;
+      if (gm < M && gk < K) va = ta ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk];
+      As[kk][mm] = va;
+      const int nn = e / 16;
+      const int gn = n0 + nn;
+      double vb = 0.0;
+      if (gn < N && gk < K) vb = tb ? B[(long)gn * ldb + gk] : B[(long)gk * ldb + gn];
+      Bs[kk][nn] = vb;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      double av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gm = m0 + ty + 16 * i, gn = n0 + tx + 16 * j;
+      if (gm < M && gn < N) C[(long)gm * ldc + gn] = acc[i][j];
+    }
+}
+
+static void gemm(int batch, int M, int N, int K, const double* A, long lda, long sA, int ta,
+                 const double* B, long ldb, long sB, int tb, double* C, long ldc, long sC,
+                 hipStream_t s) {
+  dim3 g((N + 63) / 64, (M + 63) / 64, batch);
+  hipLaunchKernelGGL(gemm_f64, g, dim3(256), 0, s, M, N, K, A, lda, sA, ta, B, ldb, sB, tb, C, ldc,
+                     sC);
+}
+
+// average of the permuted Cartesian matrix over the translations
+__global__ void perm_average(const double* __restrict__ Ccart, const int* __restrict__ inv_perms,
+                             int n_cart, int n_cells, double* __restrict__ Cavg) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nn = (long)n_cart * n_cart;
+  if (e >= nn) return;
+  const int J = (int)(e / n_cart), K = (int)(e % n_cart);
+  const double* C = Ccart + chain * nn;
+  double s = 0.0;
+  for (int t = 0; t < n_cells; ++t) {
+    const int* ip = inv_perms + (long)t * n_cart;
+    s += C[(long)ip[J] * n_cart + ip[K]];
+  }
+  Cavg[chain * nn + e] = s / n_cells;
+}
+
+// f <- P_T f  (P_T = mean of the mode-space translation operators)
+__global__ void project_f(const double* __restrict__ tproj, const double* __restrict__ f_all, int n,
+                          double* __restrict__ out_all) {
+  const int chain = blockIdx.y;
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const double* f = f_all + (long)chain * n;
+  double s = 0.0;
+  for (int k = 0; k < n; ++k) s += tproj[(long)m * n + k] * f[k];
+  out_all[(long)chain * n + m] = s;
+}
+
+int project_translations(Ctx* c, int nc, hipStream_t s) {
+  const int n = c->n, nc_ = c->n_cart;
+  // mode_tmp = pols * d2v            (n_cart x n)
+  gemm(nc, nc_, n, n, c->pols, n, 0, 0, c->d2v_fin, n, (long)n * n, 0, c->mode_tmp, n,
+       (long)nc_ * n, s);
+  // cart_tmp = mode_tmp * pols^T     (n_cart x n_cart)
+  gemm(nc, nc_, nc_, n, c->mode_tmp, n, (long)nc_ * n, 0, c->pols, n, 0, 1, c->cart_tmp, nc_,
+       (long)nc_ * nc_, s);
+  const long nn = (long)nc_ * nc_;
+  hipLaunchKernelGGL(perm_average, dim3((unsigned)((nn + 255) / 256), nc), dim3(256), 0, s,
+                     c->cart_tmp, c->inv_perms, nc_, c->n_cells, c->cart_tmp2);
+  // mode_tmp = pols^T * Cavg         (n x n_cart)
+  gemm(nc, n, nc_, nc_, c->pols, n, 0, 1, c->cart_tmp2, nc_, nn, 0, c->mode_tmp, nc_,
+       (long)n * nc_, s);
+  // d2v = mode_tmp * pols            (n x n)
+  gemm(nc, n, n, nc_, c->mode_tmp, nc_, (long)n * nc_, 0, c->pols, n, 0, 0, c->d2v_fin, n,
+       (long)n * n, s);
+  hipLaunchKernelGGL(project_f, dim3((n + 127) / 128, nc), dim3(128), 0, s, c->tproj, c->f_fin, n,
+                     c->f_proj);
+  return 0;
+}
+
+// out = [L1 + L_anh](psi), then reverse/shift
+__global__ void assemble_out(const double* __restrict__ psi_all, long len, int n,
+                             const double* __restrict__ w, const double* __restrict__ f_all,
+                             const double* __restrict__ d2v_all, const int* __restrict__ prow,
+                             const int* __restrict__ pcol, const double* __restrict__ sm,
+                             const double* __restrict__ sp, const double* __restrict__ ha,
+                             const double* __restrict__ hb, const double* __restrict__ sc,
+                             const double* __restrict__ yw, const double* __restrict__ reaw,
+                             int use_wigner, int transpose, int harmonic, int anharmonic,
+                             int reverse, double shift, double* __restrict__ out_all) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len) return;
+  const double* psi = psi_all + chain * len;
+  const double* f = f_all + (long)chain * n;
+  const double* d2v = d2v_all + (long)chain * n * n;
+  const long P = (long)n * (n + 1) / 2;
+  double v = 0.0;
+  if (e < n) {
+    const double wm = w[e];
+    if (harmonic) v = use_wigner ? -(psi[e] * wm) * wm : (psi[e] * wm) * wm;
+    if (anharmonic) v += use_wigner ? f[e] : -f[e];
+  } else {
+    const bool secA = e < n + P;
+    const long p = secA ? e - n : e - n - P;
+    const int r = prow[p], col = pcol[p];
+    const double pa = psi[n + p], pb = psi[n + P + p];
+    if (use_wigner) {
+      if (harmonic) v = secA ? ha[p] * pa : hb[p] * pb;
+      if (anharmonic) {
+        const double d = d2v[(long)r * n + col];
+        v += secA ? sm[p] * d : -sp[p] * d;
+      }
+    } else {
+      if (harmonic) {
+        const double X = sc[SC_X * P + p], Y = sc[SC_Y * P + p];
+        const double X1 = sc[SC_X1 * P + p], Y1 = sc[SC_Y1 * P + p];
+        if (!transpose)
+          v = secA ? (-X * pa - Y * pb) : (-X1 * pa - Y1 * pb);
+        else
+          v = secA ? (-X * pa - X1 * pb) : (-Y * pa - Y1 * pb);
+      }
+      if (anharmonic) {
+        const double d = d2v[(long)r * n + col];
+        double pert;
+        if (!transpose)
+          pert = secA ? d * (yw[r] + yw[col]) : d * (reaw[r] + reaw[col]);
+        else
+          pert = secA ? 0.5 * d / (yw[r] * yw[col]) * (r == col ? 1.0 : 2.0) : 0.0;
+        v -= pert;
+      }
+    }
+  }
+  if (reverse) v = -v;
+  out_all[chain * len + e] = v + shift * psi[e];
+}
+
+int apply_L_dev(Ctx* c, int nc, const double* psi, double* out, int use_wigner, int transpose,
+                int ignore_harmonic, int ignore_v3, int ignore_v4, int reverse, double shift,
+                int precision, hipStream_t s) {
+  int rc = setup_pairs(c);
+  if (rc) return rc;
+  const int n = c->n, np = c->np;
+  const long P = (long)n * (n + 1) / 2;
+  const long len = n + 2 * P;
+  const int anharmonic = ignore_v3 ? 0 : 1;
+  if (anharmonic) {
+    const long nn = (long)np * np;
+    hipLaunchKernelGGL(prepare_pert, dim3((unsigned)((nn + 255) / 256), nc), dim3(256), 0, s, psi,
+                       len, n, np, c->pair_ca, c->pair_cb, c->yw, c->reaw, use_wigner, transpose,
+                       c->R1, c->Y1);
+    rc = run_pa(c, nc, ignore_v4 ? 0 : 1, precision, c->f_fin, c->d2v_fin, s);
+    if (rc) return rc;
+    if (c->n_cells > 0) {
+      rc = project_translations(c, nc, s);
+      if (rc) return rc;
+    }
+  }
+  const double* f_src = (anharmonic && c->n_cells > 0) ? c->f_proj : c->f_fin;
+  hipLaunchKernelGGL(assemble_out, dim3((unsigned)((len + 255) / 256), nc), dim3(256), 0, s, psi,
+                     len, n, c->w, f_src, c->d2v_fin, c->pair_row, c->pair_col, c->pair_sm,
+                     c->pair_sp, c->pair_harm_a, c->pair_harm_b, c->std_coef, c->yw, c->reaw,
+                     use_wigner, transpose, ignore_harmonic ? 0 : 1, anharmonic, reverse, shift,
+                     out);
+  TDS_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace tdscha
+
+using namespace tdscha;
+
+extern "C" {
+
+int tdscha_set_translations(tdscha_ctx* ctx, int n_cart, const double* pols, int n_cells,
+                            const int* inv_perms, const double* trans_projector) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_set_translations: NULL context");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (n_cells <= 0) {
+    c->n_cells = 0;
+    return 0;
+  }
+  TDS_REQUIRE(pols && inv_perms && trans_projector && n_cart > 0, 2,
+              "tdscha_set_translations: NULL argument");
+  for (long e = 0; e < (long)n_cells * n_cart; ++e)
+    TDS_REQUIRE(inv_perms[e] >= 0 && inv_perms[e] < n_cart, 2,
+                "tdscha_set_translations: permutation index out of range");
+  const int n = c->n;
+  if (c->pols) {
+    hipFree(c->pols); hipFree(c->inv_perms); hipFree(c->tproj); hipFree(c->cart_tmp);
+    hipFree(c->cart_tmp2); hipFree(c->mode_tmp); hipFree(c->f_proj);
+    c->pols = nullptr;
+  }
+  c->n_cart = n_cart;
+  TDS_CHECK_HIP(hipMalloc((void**)&c->pols, sizeof(double) * (long)n_cart * n));
+  TDS_CHECK_HIP(hipMalloc((void**)&c->inv_perms, sizeof(int) * (long)n_cells * n_cart));
+  TDS_CHECK_HIP(hipMalloc((void**)&c->tproj, sizeof(double) * (long)n * n));
+  TDS_CHECK_HIP(hipMalloc((void**)&c->cart_tmp, sizeof(double) * kMaxChains * (long)n_cart * n_cart));
+  TDS_CHECK_HIP(hipMalloc((void**)&c->cart_tmp2, sizeof(double) * kMaxChains * (long)n_cart * n_cart));
+  TDS_CHECK_HIP(hipMalloc((void**)&c->mode_tmp, sizeof(double) * kMaxChains * (long)n_cart * n));
+  TDS_CHECK_HIP(hipMalloc((void**)&c->f_proj, sizeof(double) * kMaxChains * n));
+  TDS_CHECK_HIP(hipMemcpy(c->pols, pols, sizeof(double) * (long)n_cart * n, hipMemcpyHostToDevice));
+  TDS_CHECK_HIP(hipMemcpy(c->inv_perms, inv_perms, sizeof(int) * (long)n_cells * n_cart,
+                          hipMemcpyHostToDevice));
+  TDS_CHECK_HIP(hipMemcpy(c->tproj, trans_projector, sizeof(double) * (long)n * n,
+                          hipMemcpyHostToDevice));
+  c->n_cells = n_cells;
+  return 0;
+}
+
+int tdscha_apply_L_dev(tdscha_ctx* ctx, int n_vec, const double* psi_dev, double* out_dev,
+                       int use_wigner, int transpose, int ignore_harmonic, int ignore_v3,
+                       int ignore_v4, int reverse, double shift, int precision, void* stream) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_apply_L_dev: NULL context");
+  TDS_REQUIRE(n_vec >= 1 && n_vec <= kMaxChains, 2, "n_vec must be in [1, 8]");
+  TDS_REQUIRE(psi_dev && out_dev && psi_dev != out_dev, 2,
+              "tdscha_apply_L_dev: psi and out must be distinct device buffers");
+  TDS_REQUIRE(precision >= 0 && precision <= 2, 2, "precision must be 0, 1 or 2");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  hipStream_t s = stream ? reinterpret_cast<hipStream_t>(stream) : c->stream;
+  return apply_L_dev(c, n_vec, psi_dev, out_dev, use_wigner, transpose, ignore_harmonic, ignore_v3,
+                     ignore_v4, reverse, shift, precision, s);
+}
+
+int tdscha_apply_L(tdscha_ctx* ctx, int n_vec, const double* psi, double* out, int use_wigner,
+                   int transpose, int ignore_harmonic, int ignore_v3, int ignore_v4, int reverse,
+                   double shift, int precision) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_apply_L: NULL context");
+  TDS_REQUIRE(psi && out, 2, "tdscha_apply_L: NULL pointer");
+  TDS_REQUIRE(n_vec >= 1 && n_vec <= kMaxChains, 2, "n_vec must be in [1, 8]");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = setup_pairs(c);
+  if (rc) return rc;
+  const long len = c->n + (long)c->n * (c->n + 1);
+  TDS_CHECK_HIP(hipMemcpyAsync(c->psi_dev, psi, sizeof(double) * n_vec * len,
+                               hipMemcpyHostToDevice, c->stream));
+  rc = tdscha_apply_L_dev(ctx, n_vec, c->psi_dev, c->psi_out_dev, use_wigner, transpose,
+                          ignore_harmonic, ignore_v3, ignore_v4, reverse, shift, precision,
+                          c->stream);
+  if (rc) return rc;
+  TDS_CHECK_HIP(hipMemcpyAsync(out, c->psi_out_dev, sizeof(double) * n_vec * len,
+                               hipMemcpyDeviceToHost, c->stream));
+  TDS_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // extern "C"

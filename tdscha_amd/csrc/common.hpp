@@ -121,6 +121,9 @@ struct Ctx {
   // scratch for the anharmonic wrapper
   double* psi_dev = nullptr;   // [kMaxChains, len_psi] x2
   double* psi_out_dev = nullptr;
+  double* f_fin = nullptr;     // [kMaxChains, n] perturbed averages (normalised)
+  double* d2v_fin = nullptr;   // [kMaxChains, n, n]
+  double* f_proj = nullptr;    // [kMaxChains, n] translation-projected f
   double* stage_in = nullptr;  // [kMaxChains, n + n*n] host<->device staging
   double* stage_out = nullptr;
   // host-side copies of the symmetry description

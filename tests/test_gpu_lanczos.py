@@ -1,0 +1,173 @@
+"""GPU parity of the L application, the Lanczos recurrences and the continued
+fraction against the CPU oracle and the reference's golden run."""
+import os
+
+import numpy as np
+import pytest
+
+from tests.systems import GOLDEN, load_snte
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def snte():
+    return load_snte()
+
+
+def _product_from_fixture(snte, use_wigner=False, gamma=False):
+    from tdscha_amd.lanczos import Lanczos
+    lanc = Lanczos()
+    lanc.load_status(os.path.join(GOLDEN, "snte_step10.npz"))
+    lanc.use_wigner = use_wigner
+    lanc.gamma_only = gamma
+    lanc.init(use_symmetries=True, symmetries=snte["syms_gamma"] if gamma else snte["syms_full"])
+    lanc.reset()
+    return lanc
+
+
+def _oracle(snte, use_wigner=False, gamma=False):
+    from oracle import lanczos_oracle as O
+    ms = snte["syms_gamma"] if gamma else snte["syms_full"]
+    kw = {}
+    if gamma:
+        kw = dict(pols=snte["pols"], trans_projector=ms.trans_projector,
+                  inv_perms=ms.trans_cart_perms)
+    return O.OracleSystem(snte["X"], snte["Y"], snte["w"], snte["rho"], float(snte["T"]),
+                          ms.dense(), use_wigner=use_wigner, **kw)
+
+
+@pytest.mark.parametrize("use_wigner,transpose", [(False, False), (False, True), (True, False)])
+def test_apply_L_matches_oracle(snte, use_wigner, transpose):
+    lanc = _product_from_fixture(snte, use_wigner)
+    orc = _oracle(snte, use_wigner)
+    rng = np.random.default_rng(1)
+    vecs = [snte["basis_Q"][3], snte["basis_P"][5], rng.normal(size=len(snte["psi"]))]
+    for v in vecs:
+        got = lanc.apply_full_L(v.copy(), transpose=transpose)
+        ref = orc.apply_full_L(v, transpose=transpose)
+        print("wigner=%d T=%d rel %.2e" % (use_wigner, transpose, rel(got, ref)))
+        assert rel(got, ref) < 1e-12
+    # batched application equals single ones
+    B = lanc.apply_L_batch(np.array(vecs), transpose=transpose)
+    for v, bv in zip(vecs, B):
+        assert rel(bv, orc.apply_full_L(v, transpose=transpose)) < 1e-12
+    lanc.close()
+
+
+def test_apply_L_gamma_only_projection(snte):
+    lanc = _product_from_fixture(snte, True, gamma=True)
+    orc = _oracle(snte, True, gamma=True)
+    full = _oracle(snte, True, gamma=False)
+    rng = np.random.default_rng(2)
+    v = rng.normal(size=len(snte["psi"]))
+    got = lanc.apply_full_L(v.copy())
+    assert rel(got, orc.apply_full_L(v)) < 1e-12
+    # and for a Gamma vector the point-group + projector result is the full-group one
+    nat = 16
+    z = np.zeros(3 * nat)
+    z[0::6], z[3::6] = 1.0, -1.0
+    psi = np.zeros(len(snte["psi"]))
+    psi[:45] = np.einsum("a,a,ab->b", 1 / np.sqrt(snte["m"]), z, snte["pols"])
+    assert rel(lanc.apply_full_L(psi.copy()), full.apply_full_L(psi)) < 1e-11
+    lanc.close()
+
+
+def test_run_FT_reproduces_reference_golden_run(snte):
+    """Biconjugate standard-representation run of test_julia_lf.py:24-45."""
+    lanc = _product_from_fixture(snte, use_wigner=False)
+    lanc.prepare_mode(10)
+    lanc.run_FT(10, verbose=False)
+    a = np.array(lanc.a_coeffs)
+    bc = np.array(lanc.b_coeffs) * np.array(lanc.c_coeffs)
+    print("a", np.max(np.abs(a / snte["a_coeffs"] - 1)),
+          "bc", np.max(np.abs(bc / (snte["b_coeffs"] * snte["c_coeffs"]) - 1)))
+    assert np.max(np.abs(a / snte["a_coeffs"] - 1)) < 1e-10
+    assert np.max(np.abs(bc / (snte["b_coeffs"] * snte["c_coeffs"]) - 1)) < 1e-7
+    assert abs(lanc.c_coeffs[4] - 6.59744344e-07) / 6.59744344e-07 < 1e-6
+    lanc.close()
+
+
+def _rocksalt_lanczos(N=2000, seed=0, use_symmetries=False, gamma=False, L=2):
+    from tdscha_amd.lanczos import Lanczos
+    from tdscha_amd.rocksalt import Ensemble, RockSalt
+    ens = Ensemble(RockSalt(L=L), 300.0, N, seed=seed)
+    lanc = Lanczos(ens)
+    lanc.gamma_only = gamma
+    lanc.init(use_symmetries=use_symmetries)
+    return ens, lanc
+
+
+def test_device_lanczos_full_reorth_matches_oracle():
+    """configs[0] workload: 16-atom rock salt, N=2000, no symmetries, IR x
+    perturbation, Wigner; 50 steps with full reorthogonalisation.  Gate:
+    a_n, b_n within 1e-10 relative of the FP64 oracle."""
+    from oracle import lanczos_oracle as O
+    ens, lanc = _rocksalt_lanczos()
+    lanc.prepare_perturbation(ens.ir_vector(), masses_exp=-1)
+    psi0 = lanc.psi.copy()
+    chains, ms = lanc.run_FT_batched(psi0, 50, reorth=True)
+    a, b = chains[0]
+    orc = O.OracleSystem(ens.X, ens.Y, ens.w, ens.rho, ens.T, None, use_wigner=True)
+    ao, bo, _ = O.run_symmetric_full_reorth(orc.apply_full_L, psi0, 50, O.mask_dot_wigner(45))
+    n = min(len(a), len(ao))
+    print("steps", len(a), len(ao), "max rel a %.2e b %.2e" % (
+        np.max(np.abs(a[:n] / ao[:n] - 1)), np.max(np.abs(b[:n] / bo[:n] - 1))))
+    assert len(a) == len(ao) == 50
+    assert np.max(np.abs(a / ao - 1)) < 1e-10
+    assert np.max(np.abs(b / bo[:len(b)] - 1)) < 1e-10
+    lanc.close()
+
+
+def test_device_lanczos_reference_recurrence_and_batching():
+    """Without reorthogonalisation the device recurrence is the reference's
+    symmetric run_FT; batched uneven chains equal individual ones; graph
+    replay equals plain launches."""
+    from oracle import lanczos_oracle as O
+    ens, lanc = _rocksalt_lanczos(N=500, seed=3)
+    vs = []
+    for pv in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+        lanc.prepare_perturbation(ens.ir_vector(pv), masses_exp=-1)
+        vs.append(lanc.psi.copy())
+    orc = O.OracleSystem(ens.X, ens.Y, ens.w, ens.rho, ens.T, None, use_wigner=True)
+    st = O.run_FT(orc.apply_full_L, None, vs[1], 30, run_simm=True, mask=O.mask_dot_wigner(45))
+    caps = [12, 30, 21]
+    chains, _ = lanc.run_FT_batched(np.array(vs), 30, caps=caps, reorth=False)
+    chains_ng, _ = lanc.run_FT_batched(np.array(vs), 30, caps=caps, reorth=False, use_graph=False)
+    a1, b1 = chains[1]
+    print("no-reorth a rel %.2e" % np.max(np.abs(a1 / np.array(st.a) - 1)))
+    assert np.max(np.abs(a1 / np.array(st.a) - 1)) < 1e-9
+    assert np.max(np.abs(b1 / np.array(st.b) - 1)) < 1e-9
+    for (a, b), (ag, bg), cap in zip(chains, chains_ng, caps):
+        assert len(a) == cap and np.array_equal(a, ag) and np.array_equal(b, bg)
+    single, _ = lanc.run_FT_batched(vs[2], 21, reorth=False)
+    assert np.allclose(single[0][0], chains[2][0], rtol=1e-13, atol=0)
+    lanc.close()
+
+
+def test_continued_fraction_kernel_matches_oracle():
+    from oracle import lanczos_oracle as O
+    from tdscha_amd.lanczos import continued_fraction
+    rng = np.random.default_rng(5)
+    a = rng.uniform(1e-6, 4e-6, 40)
+    b = rng.uniform(1e-7, 1e-6, 40)
+    c = b * rng.uniform(0.9, 1.1, 40)
+    w = np.linspace(0, 600, 2000) / 109737.31568527
+    for kw in (dict(use_terminator=True, last_average=5, smearing=0.0, smooth_ramp=0),
+               dict(use_terminator=True, last_average=4, smearing=2e-5, smooth_ramp=3),
+               dict(use_terminator=False, last_average=1, smearing=5 / 109737.31568527,
+                    smooth_ramp=0)):
+        for wig in (True, False):
+            for cc in (c, None):
+                got = continued_fraction(a, b, cc, w, wig, **kw, perturbation_modulus=2.5)
+                ref = O.continued_fraction(a, b, [] if cc is None else cc, w, wig, **kw,
+                                           perturbation_modulus=2.5)
+                assert rel(got, ref) < 1e-12
+    # symmetric chain with len(b) = len(a) - 1 (converged)
+    got = continued_fraction(a, b[:-1], None, w, True, True, 3, 0.0)
+    ref = O.continued_fraction(a, b[:-1], [], w, True, True, 3, 0.0)
+    assert rel(got, ref) < 1e-12
