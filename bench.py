@@ -1,0 +1,228 @@
+#!/usr/bin/env python3
+"""Benchmark: L-applications/s of the batched Lanczos on one MI355X.
+
+Workload: rock-salt spring model on a 4x4x4 supercell (128 atoms, 381
+non-acoustic modes), N = 20000 configurations at 300 K, the 48 point-group
+operations as symmetry images plus the translational projector, three IR-like
+perturbations (x, y, z) run as one batch of Lanczos chains in the Wigner
+representation with full reorthogonalisation.
+
+One step = one batched Lanczos iteration = 3 L applications.  The headline
+value is the split-precision (FP16 hi/lo, FP32 accumulation; recurrence FP64)
+throughput; the FP64 path is timed on the same steps and the spectrum of the
+split path is compared with it on a 2000-point grid (0-600 cm^-1,
+5 cm^-1 broadening).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp16x2]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+RY_TO_CM = 109737.31568527
+PEAK = {"fp64": 78.6e12, "fp16x2": 2.5e15, "bf16x2": 2.5e15}
+PRODUCTS = {"fp64": 1, "fp16x2": 3, "bf16x2": 3}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(L, N, T, seed):
+    from tdscha_amd.rocksalt import Ensemble, RockSalt
+    t = time.time()
+    ens = Ensemble(RockSalt(L=L), T, N, seed=seed)
+    log("ensemble: %d atoms, %d modes, N=%d (%.1fs)" % (ens.model.nat, len(ens.w), N,
+                                                       time.time() - t))
+    return ens
+
+
+def spectrum(a, b, w_grid, eta):
+    from tdscha_amd.lanczos import continued_fraction
+    gf = continued_fraction(a, b, None, w_grid, True, use_terminator=False, smearing=eta)
+    return -np.imag(gf)
+
+
+def cpu_baseline(ens, lanc, n_sample_cfg, budget_s):
+    """Reference C kernels (oracle/_ref, compiled from LanczosFunctions.c) on a
+    bounded sample of the same workload, single thread; falls back to the
+    project's C restatement when the reference build is absent."""
+    from oracle import c_oracle as CO
+    rng = np.random.default_rng(7)
+    n = len(ens.w)
+    R1 = rng.normal(size=n)
+    Y1 = rng.normal(size=(n, n)) * 1e-3
+    Y1 = Y1 + Y1.T
+    kind = "reference" if CO.ref_lib() is not None else "port"
+    fn = CO.ref_perturb_averages if kind == "reference" else CO.oracle_perturb_averages
+    # probe with one configuration, then size the sample to the time budget
+    ns = 1
+    while True:
+        X = ens.X[:ns]
+        Y = ens.Y[:ns]
+        rho = ens.rho[:ns]
+        t = time.time()
+        fn(X, Y, ens.w, rho, R1, Y1, ens.T, 1, lanc.symmetries, lanc.degenerate_space,
+           lanc.sym_block_id)
+        dt = time.time() - t
+        if dt > budget_s / 4 or ns >= n_sample_cfg:
+            break
+        ns = min(n_sample_cfg, max(ns * 2, int(ns * budget_s / 4 / max(dt, 1e-3))))
+    images = ns * lanc.n_syms
+    total = ens.N * lanc.n_syms
+    t_full = dt * total / images
+    return {"value": 1.0 / t_full, "unit": "L-applications/s", "cores": 1, "kind": kind,
+            "sample": "%d configurations x %d symmetry images = %d of %d images, one "
+                      "perturbed-average pass (the %s C kernels), time scaled linearly in "
+                      "images (measured %.2f s)" % (ns, lanc.n_syms, images, total,
+                                                    "reference LanczosFunctions.c sym_fast"
+                                                    if kind == "reference" else
+                                                    "oracle restatement", dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp16x2")
+    ap.add_argument("--supercell", type=int, default=4)
+    ap.add_argument("--configs", type=int, default=20000)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (the ensemble sum "
+              "would shard by configuration with one all-reduce of n + n^2 doubles per L "
+              "application; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from tdscha_amd import _lib
+    from tdscha_amd.lanczos import Lanczos
+
+    ens = build_workload(args.supercell, args.configs, 300.0, seed=0)
+    lanc = Lanczos(ens)
+    lanc.gamma_only = True
+    t = time.time()
+    lanc.init(use_symmetries=True)
+    log("init (symmetries + upload + image expansion): %.1fs, n_syms=%d, images=%d" % (
+        time.time() - t, lanc.n_syms, lanc.engine.n_images))
+    psis = []
+    for pv in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+        lanc.prepare_perturbation(ens.ir_vector(pv), masses_exp=-1)
+        psis.append(lanc.psi.copy())
+    psis = np.array(psis)
+    nc = len(psis)
+    K, W = args.steps, args.warmup
+
+    def timed_run(prec, steps):
+        lanc.engine.synchronize()
+        t0 = time.perf_counter()
+        chains, dev_ms = lanc.run_FT_batched(psis, steps, reorth=True, precision=prec)
+        lanc.engine.synchronize()
+        return chains, time.perf_counter() - t0, dev_ms
+
+    results = {}
+    precs = [args.precision] + ([] if (args.no_fp64 or args.precision == "fp64") else ["fp64"])
+    for prec in precs:
+        if W > 0:
+            timed_run(prec, W)
+        chains, wall, dev_ms = timed_run(prec, K)
+        n_l = sum(len(a) for a, _ in chains)
+        results[prec] = dict(chains=chains, wall=wall, dev_ms=dev_ms, n_l=n_l)
+        log("%s: %d steps x %d chains: wall %.3fs, device loop %.3fs -> %.1f L/s" % (
+            prec, K, nc, wall, dev_ms / 1e3, n_l / wall))
+
+    head = results[args.precision]
+    value = head["n_l"] / head["wall"]
+    ms_per_step = head["wall"] / K * 1e3
+
+    # spectrum error of the headline path against the FP64 path
+    spec_err = None
+    if "fp64" in results and args.precision != "fp64":
+        w_grid = np.linspace(0.0, 600.0, 2000) / RY_TO_CM
+        eta = 5.0 / RY_TO_CM
+        errs = []
+        for (a1, b1), (a0, b0) in zip(head["chains"], results["fp64"]["chains"]):
+            s1 = spectrum(a1, b1, w_grid, eta)
+            s0 = spectrum(a0, b0, w_grid, eta)
+            errs.append(float(np.max(np.abs(s1 - s0)) / np.max(np.abs(s0))))
+        spec_err = max(errs)
+        log("spectrum L-inf relative error vs FP64: %s" % errs)
+
+    # per-kernel durations with HIP events on the library stream
+    L = _lib.lib()
+    prof = {}
+    for prec in results:
+        ms = np.zeros(4)
+        rc = L.tdscha_profile_perturb_averages(lanc.engine.ctx, nc,
+                                               {"fp64": 0, "fp16x2": 1, "bf16x2": 2}[prec], 5,
+                                               C.c_void_p(ms.ctypes.data))
+        _lib.check(rc, "profile")
+        prof[prec] = ms
+        log("%s kernels (ms per batched pass): weights %.3f outer %.3f reduce %.3f total %.3f"
+            % (prec, *ms))
+    n = lanc.n_modes
+    M = lanc.engine.n_images
+    pm = prof[args.precision]
+    dom = int(np.argmax(pm[:2]))
+    kname = ["weights_gemm", "outer_gemm"][dom]
+    alg = 2.0 * M * n * n * nc                 # one contraction of 2 M n^2 per chain
+    achieved = alg * PRODUCTS[args.precision] / (pm[dom] * 1e-3)
+    roof = {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12,
+            "peak": PEAK[args.precision] / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK[args.precision], "traffic": None}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ens, lanc, 64, args.cpu_budget)
+        log("cpu baseline: %s" % cpu)
+
+    dtype = {"fp64": "f64", "fp16x2": "f16x2-split/f32-acc + f64 recurrence",
+             "bf16x2": "bf16x2-split/f32-acc + f64 recurrence"}[args.precision]
+    out = {
+        "metric": "L-applications/s at 128 atoms x 20k configs x 48 syms; split-prec vs FP64, "
+                  "spec err",
+        "value": value,
+        "unit": "L-applications/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded rock-salt spring-model ensemble)",
+        "config": {"workload": "rock-salt spring model %dx%dx%d supercell (%d atoms, %d modes), "
+                               "%d configurations, %d point-group images + translational "
+                               "projector, 3 IR chains, Wigner, full reorthogonalisation" % (
+                                   args.supercell, args.supercell, args.supercell,
+                                   ens.model.nat, n, args.configs, lanc.n_syms),
+                   "chains": nc, "images": M, "precision": args.precision},
+        "fp64_value": (results["fp64"]["n_l"] / results["fp64"]["wall"]) if "fp64" in results
+        else None,
+        "speedup_vs_fp64": (value / (results["fp64"]["n_l"] / results["fp64"]["wall"]))
+        if ("fp64" in results and args.precision != "fp64") else None,
+        "spectrum_linf_rel_err": spec_err,
+        "kernel_ms": {p: [float(x) for x in prof[p]] for p in prof},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    lanc.close()
+
+
+if __name__ == "__main__":
+    main()

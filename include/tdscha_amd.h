@@ -140,6 +140,15 @@ int tdscha_continued_fraction(int n_a, const double* a, int n_b, const double* b
                               double* gf_out);
 
 /*
+ * Time the perturbed-average pipeline with HIP events on the context stream:
+ * reps passes (after one warm-up) for n_chains chains at the given precision.
+ * ms_out[4] = average ms of {weights GEMM, outer-product GEMM, K-split
+ * reduction, total}.
+ */
+int tdscha_profile_perturb_averages(tdscha_ctx* ctx, int n_chains, int precision, int reps,
+                                    double* ms_out);
+
+/*
  * Device memory helpers, so callers need no other GPU runtime for buffer
  * handoff.  kind: 0 host->host, 1 host->device, 2 device->host, 3 device->device.
  */

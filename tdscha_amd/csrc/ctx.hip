@@ -16,8 +16,8 @@ const char* last_error() { return g_err.c_str(); }
 __global__ void expand_images(const double*, const double*, const double*, const int*, const int*,
                               const int*, const int*, const long*, const int*, const double*, int,
                               int, int, long, double*, double*, double*);
-int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s);
-int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s);
+int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* ev);
+int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s, hipEvent_t* ev);
 
 // pad R1 (n) -> (np) and Y1 (n,n) -> symmetric (np,np); zero outside
 __global__ void pad_inputs(const double* __restrict__ R1, const double* __restrict__ Y1, int n,
@@ -66,9 +66,9 @@ static int choose_ksplit(int ntile, long Mp) {
 int run_pa(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2v_dev, hipStream_t s) {
   int rc;
   if (prec == PREC_FP64) {
-    rc = launch_pa_fp64(c, nc, apply_d4, s);
+    rc = launch_pa_fp64(c, nc, apply_d4, s, nullptr);
   } else {
-    rc = launch_pa_split(c, nc, apply_d4, prec, s);
+    rc = launch_pa_split(c, nc, apply_d4, prec, s, nullptr);
   }
   if (rc) return rc;
   const double norm = 1.0 / (c->n_eff * c->n_syms);
@@ -346,6 +346,53 @@ int tdscha_device_count(int* count) {
   TDS_REQUIRE(count != nullptr, 2, "tdscha_device_count: NULL out");
   *count = 0;
   if (hipGetDeviceCount(count) != hipSuccess) *count = 0;
+  return 0;
+}
+
+__global__ void fill_pattern(double* R1, double* Y1, int n, int np, int nc) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long nn = (long)np * np;
+  if (e >= nn * nc) return;
+  const int chain = (int)(e / nn);
+  const long q = e % nn;
+  const int a = (int)(q / np), b = (int)(q % np);
+  Y1[e] = (a < n && b < n) ? 1e-3 * cos(0.37 * (a + b) + 0.11 * a * b + chain) : 0.0;
+  if (q < np) R1[(long)chain * np + q] = (q < n) ? sin(0.5 * q + chain) : 0.0;
+}
+
+int tdscha_profile_perturb_averages(tdscha_ctx* ctx, int n_chains, int precision, int reps,
+                                    double* ms_out) {
+  TDS_REQUIRE(ctx != nullptr && ms_out != nullptr, 2, "tdscha_profile_perturb_averages: NULL");
+  TDS_REQUIRE(n_chains >= 1 && n_chains <= kMaxChains && reps >= 1, 2, "bad n_chains / reps");
+  TDS_REQUIRE(precision >= 0 && precision <= 2, 2, "precision must be 0, 1 or 2");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  hipStream_t s = c->stream;
+  if (precision != PREC_FP64) {
+    int rc = ensure_split(c, precision);
+    if (rc) return rc;
+  }
+  const long tot = (long)c->np * c->np * n_chains;
+  hipLaunchKernelGGL(fill_pattern, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, c->R1,
+                     c->Y1, c->n, c->np, n_chains);
+  hipEvent_t ev[4];
+  for (int i = 0; i < 4; ++i) TDS_CHECK_HIP(hipEventCreate(&ev[i]));
+  double acc[4] = {0, 0, 0, 0};
+  for (int r = 0; r < reps + 1; ++r) {
+    int rc = precision == PREC_FP64 ? launch_pa_fp64(c, n_chains, 1, s, ev)
+                                    : launch_pa_split(c, n_chains, 1, precision, s, ev);
+    if (rc) return rc;
+    TDS_CHECK_HIP(hipStreamSynchronize(s));
+    if (r == 0) continue;  // first pass is a warm-up
+    float t01, t12, t23;
+    (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
+    (void)hipEventElapsedTime(&t12, ev[1], ev[2]);
+    (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
+    acc[0] += t01; acc[1] += t12; acc[2] += t23; acc[3] += t01 + t12 + t23;
+  }
+  for (int i = 0; i < 4; ++i) {
+    ms_out[i] = acc[i] / reps;
+    (void)hipEventDestroy(ev[i]);
+  }
   return 0;
 }
 

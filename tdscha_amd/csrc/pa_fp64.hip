@@ -306,21 +306,25 @@ __global__ void reduce_ksplit(const double* __restrict__ zpart_all,
   }
 }
 
-int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s) {
+int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* ev) {
   const int ntile = c->np / kTile;
   dim3 ga((unsigned)(c->Mp / 64), nc);
+  if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(weights_f64, ga, dim3(256), 0, s, c->U, c->F, c->Y1, c->R1, c->fy, c->fpsi,
                      c->rho_img, c->np, c->Mp, apply_d4, c->coef);
+  if (ev) (void)hipEventRecord(ev[1], s);
   const long rps = round_up_l((c->Mp + c->ksplit - 1) / c->ksplit, WB_KC);
   dim3 gb((unsigned)(ntile * ntile * c->ksplit), nc);
   hipLaunchKernelGGL(outer_f64, gb, dim3(256), 0, s, c->U, c->F, c->coef, c->fy, c->np, c->Mp,
                      c->ksplit, ntile, rps, c->zpart, c->fpart);
+  if (ev) (void)hipEventRecord(ev[2], s);
   const long nn = (long)c->np * c->np;
   const double norm = 1.0 / (c->n_eff * c->n_syms);
   // zsum -> c->d2v_out (padded), normalised f -> c->f_out
   dim3 gr((unsigned)((nn + 255) / 256), nc);
   hipLaunchKernelGGL(reduce_ksplit, gr, dim3(256), 0, s, c->zpart, c->fpart, c->np, c->ksplit,
                      c->d2v_out, c->f_out, norm);
+  if (ev) (void)hipEventRecord(ev[3], s);
   return 0;
 }
 

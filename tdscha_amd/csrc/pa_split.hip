@@ -3,8 +3,8 @@
 #include "common.hpp"
 namespace tdscha {
 int ensure_split(Ctx* c, int prec) { (void)c; (void)prec; set_error("split precision not built yet"); return 2; }
-int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s) {
-  (void)nc; (void)apply_d4; (void)s;
+int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s, hipEvent_t* ev) {
+  (void)nc; (void)apply_d4; (void)s; (void)ev;
   return ensure_split(c, prec);
 }
 }  // namespace tdscha
