@@ -93,7 +93,7 @@ def cpu_baseline(ens, lanc, n_sample_cfg, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="fp16x2")
     ap.add_argument("--supercell", type=int, default=4)

@@ -51,6 +51,14 @@ void tdscha_ctx_destroy(tdscha_ctx* ctx);
 int tdscha_ctx_info(const tdscha_ctx* ctx, int* n_modes, int* n_modes_padded, long* n_images,
                     long* n_images_padded, int* ksplit);
 
+/*
+ * Number of image ranges the outer-product GEMM is split into (a multiple of
+ * 8, one slice per XCD in turn); each range is summed in FP64 afterwards in a
+ * fixed order.  Smaller ranges shorten the FP32 accumulation of the split
+ * paths.  Re-allocates the partial-sum workspace.
+ */
+int tdscha_set_ksplit(tdscha_ctx* ctx, int ksplit);
+
 /* Wait for all work queued on the context's stream. */
 int tdscha_synchronize(tdscha_ctx* ctx);
 

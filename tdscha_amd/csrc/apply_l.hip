@@ -136,7 +136,8 @@ __global__ void finalize_pa(const double* __restrict__ zsum_all, const double* _
 }
 
 // ---------------------------------------------------------------- small GEMM
-// C[b] = op(A[b]) * op(B[b]); row-major; 64x64 tiles, 256 threads, 4x4 per thread
+// C[b] = op(A[b]) * op(B[b]); row-major; 32x32 tiles, 256 threads, 2x2 per
+// thread, K in steps of 32 through LDS (FP64).
 __global__ __launch_bounds__(256) void gemm_f64(int M, int N, int K, const double* __restrict__ A,
                                                 long lda, long sA, int ta,
                                                 const double* __restrict__ B, long ldb, long sB,
@@ -145,43 +146,40 @@ __global__ __launch_bounds__(256) void gemm_f64(int M, int N, int K, const doubl
   A += b * sA;
   B += b * sB;
   C += b * sC;
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
   const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  __shared__ double As[16][65];
-  __shared__ double Bs[16][65];
-  double acc[4][4] = {};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    for (int e = threadIdx.x; e < 64 * 16; e += 256) {
-      const int mm = e / 16, kk = e % 16;
-      const int gm = m0 + mm, gk = k0 + kk;
-      double va = 0.0;
-      if (gm < M && gk < K) va = ta ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk];
-      As[kk][mm] = va;
-      const int nn = e / 16;
+  __shared__ double As[32][33];
+  __shared__ double Bs[32][33];
+  double acc[2][2] = {};
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+      const int r = e >> 5, q = e & 31;
+      // A tile element (m0 + mm, k0 + kk); coalesced along the contiguous index
+      int mm = ta ? q : r, kk = ta ? r : q;
+      int gm = m0 + mm, gk = k0 + kk;
+      As[kk][mm] = (gm < M && gk < K) ? (ta ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk]) : 0.0;
+      int nn = tb ? r : q;
+      kk = tb ? q : r;
       const int gn = n0 + nn;
-      double vb = 0.0;
-      if (gn < N && gk < K) vb = tb ? B[(long)gn * ldb + gk] : B[(long)gk * ldb + gn];
-      Bs[kk][nn] = vb;
+      gk = k0 + kk;
+      Bs[kk][nn] = (gn < N && gk < K) ? (tb ? B[(long)gn * ldb + gk] : B[(long)gk * ldb + gn]) : 0.0;
     }
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      double av[4], bv[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[kk][ty + 16 * i];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tx + 16 * j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] += av[i] * bv[j];
+#pragma unroll 8
+    for (int kk = 0; kk < 32; ++kk) {
+      const double a0 = As[kk][ty], a1 = As[kk][ty + 16];
+      const double b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
+      acc[0][0] += a0 * b0;
+      acc[0][1] += a0 * b1;
+      acc[1][0] += a1 * b0;
+      acc[1][1] += a1 * b1;
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < 2; ++j) {
       const int gm = m0 + ty + 16 * i, gn = n0 + tx + 16 * j;
       if (gm < M && gn < N) C[(long)gm * ldc + gn] = acc[i][j];
     }
@@ -190,7 +188,7 @@ __global__ __launch_bounds__(256) void gemm_f64(int M, int N, int K, const doubl
 static void gemm(int batch, int M, int N, int K, const double* A, long lda, long sA, int ta,
                  const double* B, long ldb, long sB, int tb, double* C, long ldc, long sC,
                  hipStream_t s) {
-  dim3 g((N + 63) / 64, (M + 63) / 64, batch);
+  dim3 g((N + 31) / 32, (M + 31) / 32, batch);
   hipLaunchKernelGGL(gemm_f64, g, dim3(256), 0, s, M, N, K, A, lda, sA, ta, B, ldb, sB, tb, C, ldc,
                      sC);
 }
@@ -212,15 +210,16 @@ __global__ void perm_average(const double* __restrict__ Ccart, const int* __rest
   Cavg[chain * nn + e] = s / n_cells;
 }
 
-// f <- P_T f  (P_T = mean of the mode-space translation operators)
-__global__ void project_f(const double* __restrict__ tproj, const double* __restrict__ f_all, int n,
-                          double* __restrict__ out_all) {
+// f <- P_T f  (P_T = mean of the mode-space translation operators); tprojT
+// holds P_T transposed so that consecutive threads read consecutive entries
+__global__ void project_f(const double* __restrict__ tprojT, const double* __restrict__ f_all,
+                          int n, double* __restrict__ out_all) {
   const int chain = blockIdx.y;
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
   const double* f = f_all + (long)chain * n;
   double s = 0.0;
-  for (int k = 0; k < n; ++k) s += tproj[(long)m * n + k] * f[k];
+  for (int k = 0; k < n; ++k) s += tprojT[(long)k * n + m] * f[k];
   out_all[(long)chain * n + m] = s;
 }
 
@@ -370,7 +369,10 @@ int tdscha_set_translations(tdscha_ctx* ctx, int n_cart, const double* pols, int
   TDS_CHECK_HIP(hipMemcpy(c->pols, pols, sizeof(double) * (long)n_cart * n, hipMemcpyHostToDevice));
   TDS_CHECK_HIP(hipMemcpy(c->inv_perms, inv_perms, sizeof(int) * (long)n_cells * n_cart,
                           hipMemcpyHostToDevice));
-  TDS_CHECK_HIP(hipMemcpy(c->tproj, trans_projector, sizeof(double) * (long)n * n,
+  std::vector<double> tT((size_t)n * n);
+  for (int a = 0; a < n; ++a)
+    for (int b = 0; b < n; ++b) tT[(size_t)b * n + a] = trans_projector[(size_t)a * n + b];
+  TDS_CHECK_HIP(hipMemcpy(c->tproj, tT.data(), sizeof(double) * (long)n * n,
                           hipMemcpyHostToDevice));
   c->n_cells = n_cells;
   return 0;

@@ -53,6 +53,10 @@ struct SplitStore {
   void* Ul = nullptr;  // [Mp, np] lo part
   void* Fh = nullptr;
   void* Fl = nullptr;
+  void* UhT = nullptr;  // [np, Mp] mode-major copies
+  void* UlT = nullptr;
+  void* FhT = nullptr;
+  void* FlT = nullptr;
   double* su = nullptr;  // [np] power-of-two column scale of U
   double* sf = nullptr;  // [np] power-of-two column scale of F
   double* umax = nullptr;  // [np] max |U| per column (unscaled)
@@ -95,7 +99,14 @@ struct Ctx {
   void* Y1l = nullptr;
   double* ysc = nullptr;    // [kMaxChains, np] column scale of split Y1
   double* bsc = nullptr;    // [kMaxChains, np] column scale of split B
-  double* cmax = nullptr;   // [kMaxChains, 2] max|c1|, max|c2|/2 (device)
+  double* cmax = nullptr;   // [kMaxChains, 2] max|c1|, max|c2| (device)
+  double* epi = nullptr;    // [kMaxChains, np, 4] weights-epilogue column factors
+  float* coef32 = nullptr;  // [kMaxChains, Mp, 4] FP32 copy of coef for the outer GEMM
+  double* wpart = nullptr;  // [kMaxChains, np/128, Mp, 4] weights row-dot partials
+  int ksplit_s = 0;          // K split of the split-precision outer GEMM
+  float* zpart32 = nullptr;  // [kMaxChains, ksplit_s, np, np] raw FP32 partial tiles
+  double* fpart_s = nullptr; // [kMaxChains, ksplit_s, np]
+  double* sbsc = nullptr;    // [kMaxChains, np] B column scales of the last call
   // Wigner / harmonic helpers
   double* pair_ca = nullptr;   // [npairs]
   double* pair_cb = nullptr;

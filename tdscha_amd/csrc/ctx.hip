@@ -129,7 +129,7 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
   c->N = n_configs;
   c->n_syms = n_syms;
   c->M = (long)n_configs * n_syms;
-  c->Mp = round_up_l(c->M, 64);
+  c->Mp = round_up_l(c->M, 128);
   c->T = T;
   double ne = 0;
   for (int i = 0; i < n_configs; ++i) ne += rho[i];
@@ -254,11 +254,12 @@ void tdscha_ctx_destroy(tdscha_ctx* ctx) {
                   c->pair_ca, c->pair_cb, c->pair_sm, c->pair_sp, c->pair_harm_a, c->pair_harm_b,
                   c->pair_row, c->pair_col, c->std_coef, c->yw, c->reaw, c->pols, c->inv_perms,
                   c->tproj, c->cart_tmp, c->cart_tmp2, c->mode_tmp, c->psi_dev, c->psi_out_dev,
-                  c->stage_in, c->stage_out, c->f_fin, c->d2v_fin, c->f_proj};
+                  c->stage_in, c->stage_out, c->f_fin, c->d2v_fin, c->f_proj, c->epi,
+                  c->coef32, c->wpart, c->zpart32, c->fpart_s, c->sbsc};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& s : c->split) {
-    void* sp[] = {s.Uh, s.Ul, s.Fh, s.Fl, s.su, s.sf, s.umax, s.fmax};
+    void* sp[] = {s.Uh, s.Ul, s.Fh, s.Fl, s.UhT, s.UlT, s.FhT, s.FlT, s.su, s.sf, s.umax, s.fmax};
     for (void* p : sp)
       if (p) hipFree(p);
   }
@@ -393,6 +394,23 @@ int tdscha_profile_perturb_averages(tdscha_ctx* ctx, int n_chains, int precision
     ms_out[i] = acc[i] / reps;
     (void)hipEventDestroy(ev[i]);
   }
+  return 0;
+}
+
+int tdscha_set_ksplit(tdscha_ctx* ctx, int ksplit) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_set_ksplit: NULL context");
+  TDS_REQUIRE(ksplit >= 8 && ksplit <= 4096 && ksplit % 8 == 0, 2,
+              "tdscha_set_ksplit: ksplit must be a multiple of 8 in [8, 4096]");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  TDS_CHECK_HIP(hipStreamSynchronize(c->stream));
+  double *zp = nullptr, *fp = nullptr;
+  TDS_CHECK_HIP(hipMalloc((void**)&zp, sizeof(double) * kMaxChains * (size_t)ksplit * c->np * c->np));
+  TDS_CHECK_HIP(hipMalloc((void**)&fp, sizeof(double) * kMaxChains * (size_t)ksplit * c->np));
+  (void)hipFree(c->zpart);
+  (void)hipFree(c->fpart);
+  c->zpart = zp;
+  c->fpart = fp;
+  c->ksplit = ksplit;
   return 0;
 }
 
