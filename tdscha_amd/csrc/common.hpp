@@ -49,18 +49,19 @@ enum Precision { PREC_FP64 = 0, PREC_FP16X2 = 1, PREC_BF16X2 = 2 };
 
 // Per-precision image storage for the split-precision kernels.
 struct SplitStore {
-  void* Uh = nullptr;  // [Mp, np] hi part of scaled images
-  void* Ul = nullptr;  // [Mp, np] lo part
-  void* Fh = nullptr;
-  void* Fl = nullptr;
-  void* UhT = nullptr;  // [np, Mp] mode-major copies
-  void* UlT = nullptr;
-  void* FhT = nullptr;
-  void* FlT = nullptr;
+  void* Uh = nullptr;   // image-major blocks, hi part of the scaled U (weights A operand)
+  void* Ul = nullptr;   //   lo part
+  void* Fh = nullptr;   // image-major blocks, hi part of the scaled F (weights epilogue)
+  void* Fl = nullptr;   //   lo part
+  void* UhT = nullptr;  // mode-major blocks, hi part of the scaled U (outer GEMM)
+  void* UlT = nullptr;  //   lo part
+  void* FhT = nullptr;  // mode-major blocks, hi part of the scaled F (outer B build)
+  void* FlT = nullptr;  //   lo part
   double* su = nullptr;  // [np] power-of-two column scale of U
   double* sf = nullptr;  // [np] power-of-two column scale of F
   double* umax = nullptr;  // [np] max |U| per column (unscaled)
   double* fmax = nullptr;  // [np] max |F| per column (unscaled)
+  float* colw = nullptr;   // [3, np] 1/su, 1/sf, fpsi in FP32 (weights epilogue)
   bool ready = false;
 };
 
@@ -97,11 +98,11 @@ struct Ctx {
   SplitStore split[3];
   void* Y1h = nullptr;      // [kMaxChains, np, np] (16-bit)
   void* Y1l = nullptr;
-  double* ysc = nullptr;    // [kMaxChains, np] column scale of split Y1
+  double* ysc = nullptr;    // [kMaxChains] power-of-two scale of split Y1
   double* bsc = nullptr;    // [kMaxChains, np] column scale of split B
   double* cmax = nullptr;   // [kMaxChains, 2] max|c1|, max|c2| (device)
-  double* epi = nullptr;    // [kMaxChains, np, 4] weights-epilogue column factors
-  float* coef32 = nullptr;  // [kMaxChains, Mp, 4] FP32 copy of coef for the outer GEMM
+  float* epi = nullptr;     // [kMaxChains, 2, np] fY R1 and R1 (weights epilogue)
+  float* coef32 = nullptr;  // [Mp/16, kMaxChains, 4, 16] FP32 coefficients for the outer GEMM
   double* wpart = nullptr;  // [kMaxChains, np/128, Mp, 4] weights row-dot partials
   int ksplit_s = 0;          // K split of the split-precision outer GEMM
   float* zpart32 = nullptr;  // [kMaxChains, ksplit_s, np, np] raw FP32 partial tiles

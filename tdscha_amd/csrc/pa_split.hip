@@ -1,5 +1,5 @@
 // Split-precision perturbed averages: FP16 (or BF16) hi/lo compensated
-// products on MFMA (v_mfma_f32_16x16x32_{f16,bf16}) with FP32 accumulation.
+// products on MFMA (v_mfma_f32_32x32x16_{f16,bf16}) with FP32 accumulation.
 //
 // Every operand x is held as x ~ (hi + lo) / s with hi = T(x s), lo =
 // T(x s - hi) (T = fp16 or bf16) and s a power-of-two per column, so a
@@ -8,15 +8,17 @@
 // The per-image coefficients (c1, c2, alpha, beta) and all reductions across
 // work-groups stay FP64.
 //
-// Storage (built once per context and precision):
-//   Uh, Ul, Fh, Fl     [Mp, np]  row-major images       (weights GEMM, K = modes)
-//   UhT, UlT, FhT, FlT [np, Mp]  mode-major images      (outer GEMM, K = images)
-// so that both GEMMs read K-contiguous 16-byte fragments: the MFMA 16x16x32
-// operand layout is A[i][k]: i = lane % 16, k = 8 * (lane / 16) + 0..7 (same
-// for B with j), result D[i][j]: j = lane % 16, i = 4 * (lane / 16) + r.
+// Operand storage is blocked: every GEMM operand is a sequence of 4 KB
+// blocks of 128 rows x 16 K-elements, each block byte-for-byte the LDS image
+// the MFMA fragment reads want (16-byte chunks XOR-swizzled by row).  A block
+// is therefore copied global -> LDS with four 1 KB direct-to-LDS loads
+// (global_load_lds_dwordx4), fully coalesced, without passing through VGPRs.
+//   UA, FA   [Mp/128][np/16] blocks    image-major (weights GEMM, K = modes)
+//   UT, FT   [np/128][Mp/16] blocks    mode-major  (outer GEMM,   K = images)
+//   YB       [chain][np/128][np/16]    Y1 columns  (weights GEMM B operand)
 //
-// All chains of one batch are processed inside a work-group, so each image
-// tile is read from HBM once per batch (template NC = chains per pass).
+// All chains of one pass (up to 3) share a work-group, so every image block
+// is read once per pass.
 #include <cmath>
 #include <type_traits>
 
@@ -51,6 +53,7 @@ struct Mma<__bf16> {
 
 typedef unsigned short u16;
 typedef u16 u16x8 __attribute__((ext_vector_type(8)));
+typedef u16 u16x4 __attribute__((ext_vector_type(4)));
 
 template <typename H>
 __device__ __forceinline__ float to_f(u16 b) {
@@ -86,6 +89,45 @@ __device__ __forceinline__ void atomic_max_pos(double* addr, double v) {
   atomicMax(reinterpret_cast<unsigned long long*>(addr), (unsigned long long)__double_as_longlong(v));
 }
 
+// ------------------------------------------------------------ block layout
+constexpr int kBR = 128;            // rows of a block
+constexpr int kBK = 16;             // K-elements of a block (one 32x32x16 step)
+constexpr int kBE = kBR * kBK;      // 16-bit elements of a block (4 KB)
+// Chunk-major: the block is two 2 KB halves (K-elements 0..7 and 8..15), each
+// 128 rows x 16 bytes.  An MFMA fragment read (32 consecutive rows, one half)
+// is then 512 contiguous bytes per half-wave, free of bank conflicts, and a
+// linear 16-byte-per-lane pass over a block gives every wave one half.
+__host__ __device__ __forceinline__ int blk_off(int row, int k) {
+  return (k >> 3) * (kBR * 8) + row * 8 + (k & 7);
+}
+// fragment of rows 32 b .. 32 b + 31: lane reads row 32 b + lane % 32, half lane / 32
+__device__ __forceinline__ int frag_off(int b, int lane) {
+  return (lane >> 5) * (kBR * 8) + (32 * b + (lane & 31)) * 8;
+}
+
+// direct global -> LDS copy of 16 bytes per lane: lane i's bytes land at
+// lds_base + 16 i (lds_base is wave-uniform)
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+typedef __attribute__((address_space(1))) void* gbl_ptr_t;
+__device__ __forceinline__ void dma16(const void* g, void* lds_base) {
+  __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, (lds_ptr_t)lds_base, 16, 0, 0);
+}
+
+// s_waitcnt vmcnt(N) (expcnt and lgkmcnt not waited), gfx9 encoding
+#define TDS_WAIT_VMCNT(N) \
+  __builtin_amdgcn_s_waitcnt(((N) & 15) | (((N) >> 4) << 14) | 0x0F70)
+
+// Work-group barrier without a memory fence.  The direct-to-LDS copies are
+// completed explicitly (vmcnt) before it; this waits for the wave's own LDS
+// reads and writes (lgkmcnt(0)), and the empty asm statements keep the
+// compiler from moving memory accesses across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // ------------------------------------------------------------ static prep
 __global__ void colmax_kernel(const double* __restrict__ U, const double* __restrict__ F, long Mp,
                               int np, double* __restrict__ umx_out, double* __restrict__ fmx_out) {
@@ -112,81 +154,96 @@ __global__ void scales_kernel(const double* __restrict__ umx, const double* __re
   sf[col] = pow2_scale(fmx[col]);
 }
 
-// row-major hi/lo and the mode-major copies, through a 32x32 LDS transpose
+// hi/lo split of the images into the image-major and mode-major block
+// layouts (32 x 32 tiles through LDS for the transpose)
 template <typename H>
 __global__ void split_images(const double* __restrict__ U, const double* __restrict__ F, long Mp,
                              int np, const double* __restrict__ su,
-                             const double* __restrict__ sf, u16* __restrict__ Uh,
-                             u16* __restrict__ Ul, u16* __restrict__ Fh, u16* __restrict__ Fl,
-                             u16* __restrict__ UhT, u16* __restrict__ UlT,
-                             u16* __restrict__ FhT, u16* __restrict__ FlT) {
+                             const double* __restrict__ sf, u16* __restrict__ UAh,
+                             u16* __restrict__ UAl, u16* __restrict__ FAh, u16* __restrict__ FAl,
+                             u16* __restrict__ UTh, u16* __restrict__ UTl,
+                             u16* __restrict__ FTh, u16* __restrict__ FTl) {
   __shared__ u16 t[4][32][33];
   const long r0 = (long)blockIdx.x * 32;
   const int c0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const int nkc = np / kBK;
   for (int yy = ty; yy < 32; yy += 8) {
     const long r = r0 + yy;
-    const int cc = c0 + tx;
-    const double u = U[r * np + cc] * su[cc];
-    const double f = F[r * np + cc] * sf[cc];
+    const int m = c0 + tx;
+    const double u = U[r * np + m] * su[m];
+    const double f = F[r * np + m] * sf[m];
     const u16 uh = from_d<H>(u), fh = from_d<H>(f);
     const u16 ul = from_d<H>(u - (double)to_f<H>(uh));
     const u16 fl = from_d<H>(f - (double)to_f<H>(fh));
-    Uh[r * np + cc] = uh;
-    Ul[r * np + cc] = ul;
-    Fh[r * np + cc] = fh;
-    Fl[r * np + cc] = fl;
+    const long a = ((r / kBR) * nkc + m / kBK) * kBE + blk_off((int)(r % kBR), m % kBK);
+    UAh[a] = uh;
+    UAl[a] = ul;
+    FAh[a] = fh;
+    FAl[a] = fl;
     t[0][yy][tx] = uh;
     t[1][yy][tx] = ul;
     t[2][yy][tx] = fh;
     t[3][yy][tx] = fl;
   }
   __syncthreads();
+  const long nki = Mp / kBK;
   for (int yy = ty; yy < 32; yy += 8) {
-    const long dst = (long)(c0 + yy) * Mp + r0 + tx;
-    UhT[dst] = t[0][tx][yy];
-    UlT[dst] = t[1][tx][yy];
-    FhT[dst] = t[2][tx][yy];
-    FlT[dst] = t[3][tx][yy];
+    const int m = c0 + yy;
+    const long r = r0 + tx;
+    const long a = ((long)(m / kBR) * nki + r / kBK) * kBE + blk_off(m % kBR, (int)(r % kBK));
+    UTh[a] = t[0][tx][yy];
+    UTl[a] = t[1][tx][yy];
+    FTh[a] = t[2][tx][yy];
+    FTl[a] = t[3][tx][yy];
   }
 }
 
 // ------------------------------------------------------------ per-call prep
-// YT[c][j][k] = Y1[c][j][k] sy[j] / su[k] split hi/lo; epilogue factors;
-// resets the |c| maxima.  One block per (row j, chain).
-template <typename H>
-__global__ void prep_y(const double* __restrict__ Y1all, const double* __restrict__ R1all,
-                       const double* __restrict__ su, const double* __restrict__ sf,
-                       const double* __restrict__ fy, const double* __restrict__ fpsi, int np,
-                       u16* __restrict__ YhT, u16* __restrict__ YlT, double* __restrict__ epi,
-                       double* __restrict__ cmax) {
-  const int j = blockIdx.x, chain = blockIdx.y;
-  const double* Y = Y1all + ((long)chain * np + j) * np;
+// One power-of-two scale per chain for the split Y1: sy with
+// max_jk |Y1[j][k] / su[k]| * sy < 2^12.  One block per chain.
+__global__ void ymax_kernel(const double* __restrict__ Y1all, const double* __restrict__ su,
+                            int np, double* __restrict__ ysc) {
+  const int chain = blockIdx.x;
+  const double* Y = Y1all + (long)chain * np * np;
   __shared__ double red[256];
   double mx = 0.0;
-  for (int k = threadIdx.x; k < np; k += blockDim.x) mx = fmax(mx, fabs(Y[k] / su[k]));
+  for (long e = threadIdx.x; e < (long)np * np; e += blockDim.x)
+    mx = fmax(mx, fabs(Y[e] / su[e % np]));
   red[threadIdx.x] = mx;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
     __syncthreads();
   }
-  const double sy = pow2_scale(red[0]);
-  u16* yh = YhT + ((long)chain * np + j) * np;
-  u16* yl = YlT + ((long)chain * np + j) * np;
+  if (threadIdx.x == 0) ysc[chain] = pow2_scale(red[0]);
+}
+
+// YB[c] block (column tile, K step) holds Y1[c][j][k] sy / su[k] split hi/lo;
+// per-chain epilogue column factors (fY R1, R1) in FP32; resets the |c|
+// maxima.  One block per (column j, chain).
+template <typename H>
+__global__ void prep_y(const double* __restrict__ Y1all, const double* __restrict__ R1all,
+                       const double* __restrict__ su, const double* __restrict__ fy, int np,
+                       const double* __restrict__ ysc, u16* __restrict__ YBh,
+                       u16* __restrict__ YBl, float* __restrict__ epw,
+                       double* __restrict__ cmax) {
+  const int j = blockIdx.x, chain = blockIdx.y;
+  const double* Y = Y1all + ((long)chain * np + j) * np;
+  const double sy = ysc[chain];
+  const int nct = np / kBR, nkc = np / kBK;
+  const long base = ((long)chain * nct + j / kBR) * nkc * kBE;
   for (int k = threadIdx.x; k < np; k += blockDim.x) {
     const double v = Y[k] * sy / su[k];
     const u16 h = from_d<H>(v);
-    yh[k] = h;
-    yl[k] = from_d<H>(v - (double)to_f<H>(h));
+    const long a = base + (long)(k / kBK) * kBE + blk_off(j % kBR, k % kBK);
+    YBh[a] = h;
+    YBl[a] = from_d<H>(v - (double)to_f<H>(h));
   }
   if (threadIdx.x == 0) {
     const double R = R1all[(long)chain * np + j];
-    double* e = epi + ((long)chain * np + j) * 4;
-    e[0] = 1.0 / (sy * su[j]);
-    e[1] = fpsi[j] / (sy * sf[j]);
-    e[2] = fy[j] * R / su[j];
-    e[3] = R / sf[j];
+    epw[((long)chain * 2 + 0) * np + j] = (float)(fy[j] * R);
+    epw[((long)chain * 2 + 1) * np + j] = (float)R;
     if (j == 0) {
       cmax[chain * 2 + 0] = 0.0;
       cmax[chain * 2 + 1] = 0.0;
@@ -194,204 +251,269 @@ __global__ void prep_y(const double* __restrict__ Y1all, const double* __restric
   }
 }
 
-// ----------------------------------------------------------- LDS layout
-// Tiles are 128 rows x 32 (16-bit) elements: four 16-byte chunks per row,
-// unpadded, chunk index XOR-swizzled with (row / 4) % 4.  Measured on gfx950
-// this makes both the 32x32x16 fragment reads (32 consecutive rows x one chunk
-// per half-wave) and the staging writes (4 chunks per row) conflict-free.
-__device__ __forceinline__ int lds_off(int row, int chunk) {
-  return row * 32 + 8 * (chunk ^ ((row >> 2) & 3));
+// chain-independent epilogue column factors: 1/su, 1/sf, fpsi
+__global__ void colw_kernel(const double* __restrict__ su, const double* __restrict__ sf,
+                            const double* __restrict__ fpsi, int np, float* __restrict__ colw) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  colw[j] = (float)(1.0 / su[j]);
+  colw[np + j] = (float)(1.0 / sf[j]);
+  colw[2 * np + j] = (float)fpsi[j];
 }
-constexpr int kTileElems = 128 * 32;  // u16 elements of one 128 x 32 tile
 
 // ----------------------------------------------------------- weights GEMM
-// tile 128 rows x 128 cols, 4 waves (2 x 2) of 64 x 64, one wave per SIMD;
-// K = np in chunks of 32, double-buffered LDS, one barrier per chunk
-constexpr int SW_BM = 128, SW_BN = 128, SW_KC = 32;
+// V_c = U Y1_c for a tile of 128 images x 128 modes per chain, then per image
+// the row-dot products s1 = V.u, s2 = V.(fpsi f), s3 = u.(fY R1), s4 = f.R1
+// over the tile's columns (partials per column tile).
+// Persistent: the grid is 8 x nper work-groups; XCD x (= blockIdx % 8) owns
+// the row tiles rt = x (mod 8), so the column tiles of one row tile run
+// side by side on one L2.  Each work-group walks its tiles as one flat
+// stream of 16-wide K steps through a ring of 4 LDS slots (separate arrays,
+// slot index fixed at compile time by a 4-fold unroll), direct-to-LDS loads
+// issued 3 steps ahead.  Wave w owns the 32 columns 32 w .. 32 w + 31 of the
+// tile for all 128 images (4 x 1 blocks of 32 x 32 per chain): per step it
+// reads 8 image fragments and 2 NC Y1 fragments, one step ahead of its MFMAs.
+template <int K>
+using ic = std::integral_constant<int, K>;
 
 template <typename H, int NC>
 __global__ __launch_bounds__(256, 1) void weights_split(
-    const u16* __restrict__ Uh, const u16* __restrict__ Ul, const u16* __restrict__ Fh,
-    const u16* __restrict__ Fl, const u16* __restrict__ YhT, const u16* __restrict__ YlT,
-    const double* __restrict__ epi, int np, long Mp, int chain0, int nrt,
-    double* __restrict__ part) {
+    const u16* __restrict__ UAh, const u16* __restrict__ UAl, const u16* __restrict__ FAh,
+    const u16* __restrict__ FAl, const u16* __restrict__ YBh, const u16* __restrict__ YBl,
+    const float* __restrict__ colw, const float* __restrict__ epw, int np, long Mp, int chain0,
+    int nrt, double* __restrict__ part) {
   typedef typename Mma<H>::vec vec;
-  const int nct = np / SW_BN;
-  // XCD-aware: all column tiles of a row tile land on the same XCD
-  const int id = blockIdx.x;
-  const int xcd = id & 7, slot = id >> 3;
-  const int ct = slot % nct;
-  const int rt = xcd + 8 * (slot / nct);
-  if (rt >= nrt) return;
-  const long r0 = (long)rt * SW_BM;
-  const int c0 = ct * SW_BN;
+  constexpr int kBlocks = 2 + 2 * NC;           // A hi/lo + B hi/lo per chain
+  constexpr int kSlot = kBlocks * kBE;
+  __shared__ __align__(16) u16 sw0[kSlot];
+  __shared__ __align__(16) u16 sw1[kSlot];
+  __shared__ __align__(16) u16 sw2[kSlot];
+  __shared__ __align__(16) u16 sw3[kSlot];
+  u16* const slots[4] = {sw0, sw1, sw2, sw3};
+  // during a tile's epilogue slot 3 (its fragments were read in the tile's
+  // last-but-one step; the next copies go to slots 0..2) holds the column
+  // factors and the reduction
+  constexpr int kCw = (3 + 2 * NC) * kBR;       // floats: 1/su, 1/sf, fpsi, (fY R1, R1) per chain
+  static_assert(kCw * 4 + 4 * NC * 64 * 4 * 8 <= kSlot * 2, "epilogue scratch exceeds a slot");
+  float* const cw = reinterpret_cast<float*>(sw3);
+  double* const red = reinterpret_cast<double*>(sw3 + kCw * 2);   // [wave][chain][64 rows][4]
+
+  const int nct = np / kBR, nkc = np / kBK;     // nkc is a multiple of 8
+  const int xcd = blockIdx.x & 7, s = blockIdx.x >> 3, nper = gridDim.x >> 3;
+  const int nrt_x = nrt > xcd ? (nrt - xcd + 7) / 8 : 0;
+  const int ntile_x = nrt_x * nct;
+  const int my_tiles = ntile_x > s ? (ntile_x - s + nper - 1) / nper : 0;
+  const int total = my_tiles * nkc;
+  if (total == 0) return;
+
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;  // wave rows 64*wr, cols 64*wc
+  const int lr = lane & 31, lh = lane >> 5;
 
-  constexpr int kStage = (2 + 2 * NC) * kTileElems;
-  __shared__ __align__(16) u16 smem[2 * kStage];
+  // copy step g (clamped to the stream) into slot `slot`: wave w copies
+  // bytes [1 KB w, 1 KB (w + 1)) of every block, kBlocks loads per wave
+  auto issue = [&](int g, u16* slot) {
+    if (g > total - 1) g = total - 1;
+    const int ti = g / nkc, kc = g - ti * nkc;
+    const int idx = s + ti * nper;
+    const int rt = xcd + 8 * (idx / nct), ct = idx % nct;
+    u16* dst = slot + wave * 512;
+    const long src = (long)wave * 512 + lane * 8;
+    const long ab = ((long)rt * nkc + kc) * kBE + src;
+    dma16(UAh + ab, dst);
+    dma16(UAl + ab, dst + kBE);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const long bb = (((long)(chain0 + c) * nct + ct) * nkc + kc) * kBE + src;
+      dma16(YBh + bb, dst + (2 + 2 * c) * kBE);
+      dma16(YBl + bb, dst + (3 + 2 * c) * kBE);
+    }
+  };
+  struct Frags {
+    vec a[4][2];      // image rows 32 i .., hi / lo
+    vec b[NC][2];     // Y1 columns 32 wave .., hi / lo
+  };
+  auto read = [&](const u16* cur, Frags& f) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int o = frag_off(i, lane);
+      f.a[i][0] = *reinterpret_cast<const vec*>(cur + o);
+      f.a[i][1] = *reinterpret_cast<const vec*>(cur + kBE + o);
+    }
+    const int o = frag_off(wave, lane);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      f.b[c][0] = *reinterpret_cast<const vec*>(cur + (2 + 2 * c) * kBE + o);
+      f.b[c][1] = *reinterpret_cast<const vec*>(cur + (3 + 2 * c) * kBE + o);
+    }
+  };
 
-  v16f acc[NC][2][2];
+  v16f acc[NC][4];
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[c][i][j] = v16f{};
+    for (int i = 0; i < 4; ++i) acc[c][i] = v16f{};
 
-  // staging: 128 rows x 4 chunks = 512 slots, two per thread
-  const int srow = tid >> 2, sch = tid & 3;
-  u16x8 ah[2], al[2], bh[NC][2], bl[NC][2];
-  auto load = [&](int k0) {
+  // step g (slot K % 4) with its fragments already in `cur`; reads the
+  // fragments of step g + 1 into `nxt`.  Every step issues exactly kBlocks
+  // loads per wave (the copy of step g + 3), so vmcnt(kBlocks) at its start
+  // means the copy of step g + 1 has landed.
+  auto step = [&](auto kk, auto prefetch, int g, Frags& cur, Frags& nxt) {
+    constexpr int K = decltype(kk)::value;
+    TDS_WAIT_VMCNT(kBlocks);
+    lds_barrier();
+    // slot (K + 3) % 4 was last read for step g - 1, during step g - 2 (or
+    // at its own start)
+    issue(g + 3, slots[(K + 3) & 3]);
+    if constexpr (decltype(prefetch)::value) read(slots[(K + 1) & 3], nxt);
+    // operands swapped: D = V^T block, lane holds image row 32 i + lane % 32
+    // and columns 32 wave + 8 q + 4 (lane / 32) + 0..3 (q = 0..3);
+    // product-major order keeps dependent MFMAs 4 NC instructions apart
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int row = srow + 64 * h;
-      ah[h] = *reinterpret_cast<const u16x8*>(Uh + (r0 + row) * np + k0 + 8 * sch);
-      al[h] = *reinterpret_cast<const u16x8*>(Ul + (r0 + row) * np + k0 + 8 * sch);
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const long off = ((long)(chain0 + c) * np + c0 + row) * np + k0 + 8 * sch;
-        bh[c][h] = *reinterpret_cast<const u16x8*>(YhT + off);
-        bl[c][h] = *reinterpret_cast<const u16x8*>(YlT + off);
-      }
-    }
-  };
-  auto store = [&](u16* st) {
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int o = lds_off(srow + 64 * h, sch);
-      *reinterpret_cast<u16x8*>(st + o) = ah[h];
-      *reinterpret_cast<u16x8*>(st + kTileElems + o) = al[h];
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        *reinterpret_cast<u16x8*>(st + (2 + 2 * c) * kTileElems + o) = bh[c][h];
-        *reinterpret_cast<u16x8*>(st + (3 + 2 * c) * kTileElems + o) = bl[c][h];
-      }
-    }
-  };
-  load(0);
-  store(smem);
-  __syncthreads();
-  const int nk = np / SW_KC;
-  for (int kc = 0; kc < nk; ++kc) {
-    u16* cur = smem + (kc & 1) * kStage;
-    if (kc + 1 < nk) load((kc + 1) * SW_KC);
-    // operands swapped: D = V^T block, lane holds image row 32 i + lane % 32 and
-    // columns 32 j + 8 q + 4 (lane / 32) + 0..3 (q = 0..3).  All fragment reads
-    // of the chunk are issued first; the MFMAs then run product-major so that
-    // dependent MFMAs on one accumulator are 4 NC instructions apart.
-    const int lr = lane & 31, lh = lane >> 5;
-    vec fa[2][2][2], fb[2][NC][2][2];   // [ksb][block][hi/lo]
-#pragma unroll
-    for (int ksb = 0; ksb < 2; ++ksb) {
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int o = lds_off(64 * wr + 32 * i + lr, 2 * ksb + lh);
-        fa[ksb][i][0] = *reinterpret_cast<const vec*>(cur + o);
-        fa[ksb][i][1] = *reinterpret_cast<const vec*>(cur + kTileElems + o);
-      }
+    for (int pr = 0; pr < 3; ++pr)
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int o = lds_off(64 * wc + 32 * j + lr, 2 * ksb + lh);
-          fb[ksb][c][j][0] = *reinterpret_cast<const vec*>(cur + (2 + 2 * c) * kTileElems + o);
-          fb[ksb][c][j][1] = *reinterpret_cast<const vec*>(cur + (3 + 2 * c) * kTileElems + o);
-        }
+        for (int i = 0; i < 4; ++i)
+          acc[c][i] = Mma<H>::mma(cur.b[c][pr == 1 ? 1 : 0], cur.a[i][pr == 2 ? 1 : 0], acc[c][i]);
+  };
+
+  Frags f0, f1;
+  issue(0, sw0);
+  issue(1, sw1);
+  issue(2, sw2);
+  TDS_WAIT_VMCNT(2 * kBlocks);
+  lds_barrier();
+
+  for (int ti = 0; ti < my_tiles; ++ti) {
+    // the first step's copy was certified by the last barrier; its fragments
+    // are read here, every later step's one step ahead (none across the
+    // epilogue)
+    const int gt = ti * nkc;
+    read(sw0, f0);
+    for (int kc = 0; kc < nkc - 4; kc += 4) {
+      step(ic<0>{}, std::true_type{}, gt + kc, f0, f1);
+      step(ic<1>{}, std::true_type{}, gt + kc + 1, f1, f0);
+      step(ic<2>{}, std::true_type{}, gt + kc + 2, f0, f1);
+      step(ic<3>{}, std::true_type{}, gt + kc + 3, f1, f0);
     }
+    step(ic<0>{}, std::true_type{}, gt + nkc - 4, f0, f1);
+    step(ic<1>{}, std::true_type{}, gt + nkc - 3, f1, f0);
+    step(ic<2>{}, std::true_type{}, gt + nkc - 2, f0, f1);
+    step(ic<3>{}, std::false_type{}, gt + nkc - 1, f1, f0);
+
+    // ---- epilogue of tile (rt, ct).  Per image row and chain, with u, f the
+    // unscaled images and V the accumulators (sy times the true V):
+    //   p0 = sum V u, p1 = sum V fpsi f, p2 = sum u fY R1, p3 = sum f R1
+    // over the tile's 128 columns; FP32 over 4 columns, FP64 beyond; halves
+    // of a wave combined by a shuffle, the four column waves through LDS.
+    const int idx = s + ti * nper;
+    const int rt = xcd + 8 * (idx / nct), ct = idx % nct;
+    const long r0 = (long)rt * kBR;
+    const int c0 = ct * kBR;
+    for (int e = tid; e < kCw; e += 256) {
+      const int w = e / kBR, cc = c0 + e % kBR;
+      cw[e] = w < 3 ? colw[w * np + cc] : epw[((long)chain0 * 2 + (w - 3)) * np + cc];
+    }
+    lds_barrier();
+    // two halves of 64 rows: every wave stores its column-quarter partials,
+    // then the four quarters are summed in a fixed order
 #pragma unroll
-    for (int ksb = 0; ksb < 2; ++ksb)
+    for (int half = 0; half < 2; ++half) {
 #pragma unroll
-      for (int pr = 0; pr < 3; ++pr)
+      for (int ii = 0; ii < 2; ++ii) {
+        const int i = 2 * half + ii;
+        const int row = 32 * i + lr;
+        double p[NC][4];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) p[c][0] = p[c][1] = p[c][2] = p[c][3] = 0.0;
+        // the image loads of this row first: 16 loads of 8 bytes
+        u16x4 ld[4][4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int col = 32 * wave + 8 * q + 4 * lh;
+          const long a = ((long)rt * nkc + (c0 + col) / kBK) * kBE + blk_off(row, col % kBK);
+          ld[q][0] = *reinterpret_cast<const u16x4*>(UAh + a);
+          ld[q][1] = *reinterpret_cast<const u16x4*>(UAl + a);
+          ld[q][2] = *reinterpret_cast<const u16x4*>(FAh + a);
+          ld[q][3] = *reinterpret_cast<const u16x4*>(FAl + a);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int col = 32 * wave + 8 * q + 4 * lh;   // tile column of t = 0
+          const float4 isu = *reinterpret_cast<const float4*>(cw + 0 * kBR + col);
+          const float4 isf = *reinterpret_cast<const float4*>(cw + 1 * kBR + col);
+          const float4 fps = *reinterpret_cast<const float4*>(cw + 2 * kBR + col);
+          const float su4[4] = {isu.x, isu.y, isu.z, isu.w};
+          const float sf4[4] = {isf.x, isf.y, isf.z, isf.w};
+          const float fp4[4] = {fps.x, fps.y, fps.z, fps.w};
+          float u[4], f[4], g[4];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            u[t] = (to_f<H>(ld[q][0][t]) + to_f<H>(ld[q][1][t])) * su4[t];
+            f[t] = (to_f<H>(ld[q][2][t]) + to_f<H>(ld[q][3][t])) * sf4[t];
+            g[t] = f[t] * fp4[t];
+          }
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            const float4 fr = *reinterpret_cast<const float4*>(cw + (3 + 2 * c) * kBR + col);
+            const float4 rr = *reinterpret_cast<const float4*>(cw + (4 + 2 * c) * kBR + col);
+            const float fr4[4] = {fr.x, fr.y, fr.z, fr.w};
+            const float rr4[4] = {rr.x, rr.y, rr.z, rr.w};
+            float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const float v = acc[c][i][4 * q + t];
+              q0 += v * u[t];
+              q1 += v * g[t];
+              q2 += u[t] * fr4[t];
+              q3 += f[t] * rr4[t];
+            }
+            p[c][0] += (double)q0;
+            p[c][1] += (double)q1;
+            p[c][2] += (double)q2;
+            p[c][3] += (double)q3;
+          }
+        }
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int j = 0; j < 2; ++j)
+          for (int qq = 0; qq < 4; ++qq) p[c][qq] += __shfl_xor(p[c][qq], 32);
+        if (lh == 0) {
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
-              acc[c][i][j] = Mma<H>::mma(fb[ksb][c][j][pr == 1 ? 1 : 0],
-                                         fa[ksb][i][pr == 2 ? 1 : 0], acc[c][i][j]);
-    if (kc + 1 < nk) store(smem + ((kc + 1) & 1) * kStage);
-    __syncthreads();
-  }
-
-  // epilogue straight from the accumulators: U and F loaded once (8 bytes per
-  // 4 columns) and shared by all chains; per-lane partials -> FP64, reduced
-  // over the two half-waves by a shuffle and over the 2 column waves in LDS.
-  double* red = reinterpret_cast<double*>(smem);  // [2 wc][NC][128 rows][4]
-  const int lr = lane & 31, lh = lane >> 5;
-  const double* e[NC];
+          for (int c = 0; c < NC; ++c)
 #pragma unroll
-  for (int c = 0; c < NC; ++c) e[c] = epi + ((long)(chain0 + c) * np + c0) * 4;
-  typedef u16 u16x4 __attribute__((ext_vector_type(4)));
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int row = 64 * wr + 32 * i + lr;
-    const long grow = r0 + row;
-    double p[NC][4];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) p[c][0] = p[c][1] = p[c][2] = p[c][3] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int col = 64 * wc + 32 * j + 8 * q + 4 * lh;
-        const u16x4 uh = *reinterpret_cast<const u16x4*>(Uh + grow * np + c0 + col);
-        const u16x4 ul = *reinterpret_cast<const u16x4*>(Ul + grow * np + c0 + col);
-        const u16x4 fh = *reinterpret_cast<const u16x4*>(Fh + grow * np + c0 + col);
-        const u16x4 fl = *reinterpret_cast<const u16x4*>(Fl + grow * np + c0 + col);
-        float u[4], f[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          u[t] = to_f<H>(uh[t]) + to_f<H>(ul[t]);
-          f[t] = to_f<H>(fh[t]) + to_f<H>(fl[t]);
-        }
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-          float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const double* ee = e[c] + (col + t) * 4;
-            const float v = acc[c][i][j][4 * q + t];
-            q0 += v * u[t] * (float)ee[0];
-            q1 += v * f[t] * (float)ee[1];
-            q2 += u[t] * (float)ee[2];
-            q3 += f[t] * (float)ee[3];
-          }
-          p[c][0] += (double)q0;
-          p[c][1] += (double)q1;
-          p[c][2] += (double)q2;
-          p[c][3] += (double)q3;
+            for (int qq = 0; qq < 4; ++qq)
+              red[((wave * NC + c) * 64 + 32 * ii + lr) * 4 + qq] = p[c][qq];
         }
       }
+      lds_barrier();
+      for (int e2 = tid; e2 < NC * 64 * 4; e2 += 256) {
+        const int c = e2 / 256, rq = e2 % 256;   // (row, quantity) of this half
+        double v = red[(0 * NC + c) * 256 + rq];
+        v += red[(1 * NC + c) * 256 + rq];
+        v += red[(2 * NC + c) * 256 + rq];
+        v += red[(3 * NC + c) * 256 + rq];
+        part[(((long)(chain0 + c) * nct + ct) * Mp + r0 + 64 * half) * 4 + rq] = v;
+      }
+      lds_barrier();
+    }
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
-      for (int qq = 0; qq < 4; ++qq) p[c][qq] += __shfl_xor(p[c][qq], 32);
-    if (lh == 0) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int qq = 0; qq < 4; ++qq) red[((wc * NC + c) * SW_BM + row) * 4 + qq] = p[c][qq];
-    }
+      for (int i = 0; i < 4; ++i) acc[c][i] = v16f{};
   }
-  __syncthreads();
-  for (int e2 = tid; e2 < NC * SW_BM; e2 += blockDim.x) {
-    const int c = e2 / SW_BM, row = e2 % SW_BM;
-    const double* a0 = red + ((0 * NC + c) * SW_BM + row) * 4;
-    const double* a1 = red + ((1 * NC + c) * SW_BM + row) * 4;
-    double* o = part + (((long)(chain0 + c) * nct + ct) * Mp + r0 + row) * 4;
-    o[0] = a0[0] + a1[0];
-    o[1] = a0[1] + a1[1];
-    o[2] = a0[2] + a1[2];
-    o[3] = a0[3] + a1[3];
-  }
+  // the copies issued past the end of the stream land before the work-group
+  // releases its LDS
+  TDS_WAIT_VMCNT(0);
 }
 
-// coefficients from the column-tile partials (fixed order); |c| maxima
+// coefficients from the column-tile partials (fixed order); |c| maxima.
+// coefB layout: [Mp/16][kMaxChains][4][16] floats (c1, c2, alpha, beta of
+// the 16 images of a K step, all chains of a step contiguous).
 __global__ void finalize_coef(const double* __restrict__ part, const double* __restrict__ rho_img,
-                              long Mp, int nct, int apply_d4, double* __restrict__ coef_all,
-                              float* __restrict__ coef32_all, double* __restrict__ cmax) {
+                              const double* __restrict__ ysc, long Mp, int nct, int apply_d4,
+                              double* __restrict__ coef_all, float* __restrict__ coefB,
+                              double* __restrict__ cmax) {
   const int chain = blockIdx.y;
   const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double m1 = 0.0, m2 = 0.0;
@@ -401,6 +523,10 @@ __global__ void finalize_coef(const double* __restrict__ part, const double* __r
       const double* p = part + (((long)chain * nct + ct) * Mp + row) * 4;
       s[0] += p[0]; s[1] += p[1]; s[2] += p[2]; s[3] += p[3];
     }
+    // the V-weighted sums carry the Y1 scale
+    const double isy = 1.0 / ysc[chain];
+    s[0] *= isy;
+    s[1] *= isy;
     const double rho = rho_img[row];
     const double c1 = rho * (-s[2] / 3.0 + apply_d4 * s[0] / 8.0);
     const double c2 = rho * (-s[3] / 3.0 + apply_d4 * s[1] / 4.0);
@@ -409,21 +535,21 @@ __global__ void finalize_coef(const double* __restrict__ part, const double* __r
       double* c = coef_all + ((long)chain * Mp + row) * 4;
       c[0] = c1; c[1] = c2; c[2] = al; c[3] = be;
     }
-    float* c32 = coef32_all + (long)chain * 4 * Mp + row;   // planes [4][Mp]
-    c32[0] = (float)c1; c32[Mp] = (float)c2; c32[2 * Mp] = (float)al; c32[3 * Mp] = (float)be;
+    float* cb = coefB + (((row / kBK) * kMaxChains + chain) * 4) * kBK + row % kBK;
+    cb[0] = (float)c1; cb[kBK] = (float)c2; cb[2 * kBK] = (float)al; cb[3 * kBK] = (float)be;
     m1 = fabs(c1);
     m2 = fabs(c2);
   }
   __shared__ double r1[256], r2[256];
   r1[threadIdx.x] = m1;
   r2[threadIdx.x] = m2;
-  __syncthreads();
+  lds_barrier();
   for (int o = 128; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
       r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + o]);
       r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + o]);
     }
-    __syncthreads();
+    lds_barrier();
   }
   if (threadIdx.x == 0) {
     atomic_max_pos(&cmax[chain * 2 + 0], r1[0]);
@@ -431,213 +557,278 @@ __global__ void finalize_coef(const double* __restrict__ part, const double* __r
   }
 }
 
+// One power-of-two B scale per chain: sb with max_nu (max|c1| fmx[nu] +
+// max|c2| fY[nu] umx[nu] / 2) * sb < 2^12, written for every nu (the
+// reduction undoes it per column).  One block per chain.
+__global__ void bscale_kernel(const double* __restrict__ cmax, const double* __restrict__ umx,
+                              const double* __restrict__ fmx, const double* __restrict__ fy,
+                              int np, double* __restrict__ sb_all) {
+  const int chain = blockIdx.x;
+  __shared__ double red[256];
+  const double a = cmax[chain * 2 + 0], b = 0.5 * cmax[chain * 2 + 1];
+  double mx = 0.0;
+  for (int nu = threadIdx.x; nu < np; nu += blockDim.x)
+    mx = fmax(mx, a * fmx[nu] + b * fy[nu] * umx[nu]);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const double sb = pow2_scale(red[0]);
+  for (int nu = threadIdx.x; nu < np; nu += blockDim.x) sb_all[(long)chain * np + nu] = sb;
+}
+
+// c1 and c2 of the coefficient blocks times the chain's B scale (exact)
+__global__ void scale_coef_kernel(const double* __restrict__ sb_all, int np, long nsub,
+                                  float* __restrict__ coefB) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;   // (step, image) pair
+  if (e >= nsub * kBK) return;
+  const float sb = (float)sb_all[(long)chain * np];
+  float* cb = coefB + ((e / kBK) * kMaxChains + chain) * 4 * kBK + e % kBK;
+  cb[0] *= sb;
+  cb[kBK] *= sb;
+}
+
 // ------------------------------------------------------------- outer GEMM
 // Z_c[mu, nu] = sum_k U[k, mu] B_c[k, nu], B_c = c1 F + (c2/2) fY U, over a
-// K range of images; tile 128 (mu) x 128 (nu), 4 waves (2 x 2) of 64 x 64,
-// double-buffered LDS.  B_c is built in FP32 from the loaded hi/lo images and
-// split again while staging.  The raw FP32 accumulators go to an FP32 partial
-// buffer (exact); the scales are undone in the FP64 reduction.
-constexpr int SO_T = 128, SO_KC = 32;
-
+// range of K steps (16 images each, a multiple of 4 steps per range); tile
+// 128 (mu) x 128 (nu).  The A blocks (U^T of the mu tile) and the raw U^T /
+// F^T blocks of the nu tile arrive by direct-to-LDS loads three steps ahead
+// in rings of 4 slots (separate arrays, slot index fixed at compile time by a
+// 4-fold unroll).  B_c of step t + 1 is built in FP32 from the raw blocks
+// while the MFMAs of step t run (each wave builds one half of the block, 8
+// images, so its coefficients are broadcast reads), split hi/lo again and stored to
+// the other of two B slots.  Wave w owns the 32 columns nu0 + 32 w .. for all
+// 128 mu (4 x 1 blocks of 32 x 32 per chain).  The raw FP32 accumulators go
+// to an FP32 partial buffer (exact); the scales are undone in the FP64
+// reduction.
 template <typename H, int NC>
 __global__ __launch_bounds__(256, 1) void outer_split(
-    const u16* __restrict__ UhT, const u16* __restrict__ UlT, const u16* __restrict__ FhT,
-    const u16* __restrict__ FlT, const float* __restrict__ coef32_all,
-    const double* __restrict__ cmax, const double* __restrict__ umx,
-    const double* __restrict__ fmx, const double* __restrict__ su,
-    const double* __restrict__ sf, const double* __restrict__ fy, int np, long Mp, int ksplit,
-    int ntile, long rows_per_split, int chain0, float* __restrict__ zpart_all,
-    double* __restrict__ fpart_all, double* __restrict__ sb_all) {
+    const u16* __restrict__ UTh, const u16* __restrict__ UTl, const u16* __restrict__ FTh,
+    const u16* __restrict__ FTl, const float* __restrict__ coefB,
+    const double* __restrict__ su, const double* __restrict__ sf, const double* __restrict__ fy,
+    int np, long Mp, int ksplit, int ntile, int chain0, float* __restrict__ zpart_all,
+    double* __restrict__ fpart_all) {
   typedef typename Mma<H>::vec vec;
+  constexpr int kA = 2 * kBE;                   // A hi/lo
+  constexpr int kR = 4 * kBE;                   // raw U^T h/l, F^T h/l
+  constexpr int kB = 2 * NC * kBE;              // B hi/lo per chain
+  __shared__ __align__(16) u16 sa0[kA];
+  __shared__ __align__(16) u16 sa1[kA];
+  __shared__ __align__(16) u16 sa2[kA];
+  __shared__ __align__(16) u16 sa3[kA];
+  __shared__ __align__(16) u16 sr0[kR];
+  __shared__ __align__(16) u16 sr1[kR];
+  __shared__ __align__(16) u16 sr2[kR];
+  __shared__ __align__(16) u16 sr3[kR];
+  __shared__ __align__(16) u16 sb0[kB];
+  __shared__ __align__(16) u16 sb1[kB];
+  constexpr int kCoef = NC * 4 * kBK;           // floats of one step's coefficients
+  __shared__ __align__(16) float sc0[kCoef];
+  __shared__ __align__(16) float sc1[kCoef];
+  __shared__ __align__(16) float sc2[kCoef];
+  __shared__ __align__(16) float sc3[kCoef];
+  u16* const sA[4] = {sa0, sa1, sa2, sa3};
+  u16* const sR[4] = {sr0, sr1, sr2, sr3};
+  u16* const sB[2] = {sb0, sb1};
+  float* const sC[4] = {sc0, sc1, sc2, sc3};
+
   const int id = blockIdx.x;
-  const int xcd = id & 7, slot = id >> 3;
+  const int xcd = id & 7, slotid = id >> 3;
   const int t2 = ntile * ntile;
-  const int ks = xcd + 8 * (slot / t2);
-  const int tile = slot % t2;
+  const int ks = xcd + 8 * (slotid / t2);
+  const int tile = slotid % t2;
+  if (ks >= ksplit) return;
   const int tm = tile / ntile, tn = tile % ntile;
-  const int mu0 = tm * SO_T, nu0 = tn * SO_T;
+  const int mu0 = tm * kBR, nu0 = tn * kBR;
   const bool diag = (tm == tn);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 31, lh = lane >> 5;
-  const int wr = wave >> 1, wc = wave & 1;
 
-  constexpr int kStage = (2 + 2 * NC) * kTileElems;
-  __shared__ __align__(16) u16 smem[2 * kStage];
+  // K steps [g0, g0 + n) in units of 4 steps
+  const long nsub = Mp / kBK, nunit = nsub / 4;
+  const long g0 = 4 * ((long)ks * nunit / ksplit);
+  const long n = 4 * ((long)(ks + 1) * nunit / ksplit) - g0;
 
-  // staging: 128 modes x 4 chunks of 8 images = 512 slots, two per thread
-  const int sn = tid >> 2, sch = tid & 3;
-  float gF[NC][2], gU[NC][2], inv_sf[2], fyu[2];
+  // builder role: thread t owns the 16 bytes at 16 t of every B block, i.e.
+  // row nu0 + t % 128, images 8 bq .. 8 bq + 7 with bq = t / 128 (one value
+  // per wave)
+  const int brow = tid & (kBR - 1);
+  const int bq = tid >> 7;
+  const int nu = nu0 + brow;
+  const float inv_sf = (float)(1.0 / sf[nu]);
+  const float hfu = (float)(0.5 * fy[nu] / su[nu]);
+
+  const long a_tile = (long)tm * nsub * kBE;
+  const long b_tile = (long)tn * nsub * kBE;
+  const long src = (long)wave * 512 + lane * 8;
+  // A and raw blocks and coefficients of step ts (clamped to the range) into
+  // ring position p: 7 loads per wave
+  auto issue = [&](long ts, int p) {
+    if (ts > n - 1) ts = n - 1;
+    const long offa = a_tile + (g0 + ts) * kBE + src;
+    const long offb = b_tile + (g0 + ts) * kBE + src;
+    u16* a = sA[p] + wave * 512;
+    u16* r = sR[p] + wave * 512;
+    dma16(UTh + offa, a);
+    dma16(UTl + offa, a + kBE);
+    dma16(UTh + offb, r);
+    dma16(UTl + offb, r + kBE);
+    dma16(FTh + offb, r + 2 * kBE);
+    dma16(FTl + offb, r + 3 * kBE);
+    // NC * 256 bytes of coefficients, a quarter per wave: lanes 0 .. 4 NC - 1
+    const float* cs = coefB + ((g0 + ts) * kMaxChains + chain0) * 4 * kBK + wave * NC * 16 + lane * 4;
+    if (lane < 4 * NC) dma16(cs, sC[p] + wave * NC * 16);
+  };
+
+  double fl_acc[NC];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int nu = nu0 + sn + 64 * h;
-    inv_sf[h] = (float)(1.0 / sf[nu]);
-    fyu[h] = (float)(fy[nu] / su[nu]);
+  for (int c = 0; c < NC; ++c) fl_acc[c] = 0.0;
+  // B of step ts (clamped) from its raw blocks; on diagonal tiles also the f
+  // partial sums (weighted by `live`, 0 for a clamped step)
+  auto build = [&](auto is_diag, const u16* rs, const float* cs, u16* b_slot, double live) {
+    constexpr bool DIAG = decltype(is_diag)::value;
+    const u16x8 r0 = *reinterpret_cast<const u16x8*>(rs + tid * 8);
+    const u16x8 r1 = *reinterpret_cast<const u16x8*>(rs + kBE + tid * 8);
+    const u16x8 r2 = *reinterpret_cast<const u16x8*>(rs + 2 * kBE + tid * 8);
+    const u16x8 r3 = *reinterpret_cast<const u16x8*>(rs + 3 * kBE + tid * 8);
+    // xf = F, yu = fY U / 2 (unscaled), shared by the chains
+    float xf[8], yu[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      yu[e] = (to_f<H>(r0[e]) + to_f<H>(r1[e])) * hfu;
+      xf[e] = (to_f<H>(r2[e]) + to_f<H>(r3[e])) * inv_sf;
+    }
+    const float* cf = cs + 8 * bq;   // wave-uniform: broadcast reads
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      // B scale per (chain, nu): sb with bound(B) * sb < 2^12
-      const double bound = cmax[(chain0 + c) * 2 + 0] * fmx[nu] +
-                           0.5 * cmax[(chain0 + c) * 2 + 1] * fy[nu] * umx[nu];
-      const double sb = pow2_scale(bound);
-      gF[c][h] = (float)(sb / sf[nu]);
-      gU[c][h] = (float)(0.5 * sb * fy[nu] / su[nu]);
-      if (ks == 0 && tm == 0 && sch == 0) sb_all[(long)(chain0 + c) * np + nu] = sb;
+      const float4 c1a = *reinterpret_cast<const float4*>(cf + (c * 4 + 0) * kBK);
+      const float4 c1b = *reinterpret_cast<const float4*>(cf + (c * 4 + 0) * kBK + 4);
+      const float4 c2a = *reinterpret_cast<const float4*>(cf + (c * 4 + 1) * kBK);
+      const float4 c2b = *reinterpret_cast<const float4*>(cf + (c * 4 + 1) * kBK + 4);
+      const float c1v[8] = {c1a.x, c1a.y, c1a.z, c1a.w, c1b.x, c1b.y, c1b.z, c1b.w};
+      const float c2v[8] = {c2a.x, c2a.y, c2a.z, c2a.w, c2b.x, c2b.y, c2b.z, c2b.w};
+      u16x8 bh, bl;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float b = c1v[e] * xf[e] + c2v[e] * yu[e];
+        const u16 hb = from_f<H>(b);
+        bh[e] = hb;
+        bl[e] = from_f<H>(b - to_f<H>(hb));
+      }
+      *reinterpret_cast<u16x8*>(b_slot + (2 * c) * kBE + tid * 8) = bh;
+      *reinterpret_cast<u16x8*>(b_slot + (2 * c + 1) * kBE + tid * 8) = bl;
+      if constexpr (DIAG) {
+        const float4 aa = *reinterpret_cast<const float4*>(cf + (c * 4 + 2) * kBK);
+        const float4 ab = *reinterpret_cast<const float4*>(cf + (c * 4 + 2) * kBK + 4);
+        const float4 ba = *reinterpret_cast<const float4*>(cf + (c * 4 + 3) * kBK);
+        const float4 bb = *reinterpret_cast<const float4*>(cf + (c * 4 + 3) * kBK + 4);
+        const float av[8] = {aa.x, aa.y, aa.z, aa.w, ab.x, ab.y, ab.z, ab.w};
+        const float bv[8] = {ba.x, ba.y, ba.z, ba.w, bb.x, bb.y, bb.z, bb.w};
+        float fsum = 0.f;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) fsum += av[e] * xf[e] + 2.0f * bv[e] * yu[e];
+        fl_acc[c] += live * (double)fsum;
+      }
     }
-  }
+  };
 
-  v16f acc[NC][2][2];
+  v16f acc[NC][4];
 #pragma unroll
   for (int c = 0; c < NC; ++c)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) acc[c][i][j] = v16f{};
+    for (int i = 0; i < 4; ++i) acc[c][i] = v16f{};
 
-  const long kb = (long)ks * rows_per_split;
-  long ke = kb + rows_per_split;
-  if (ke > Mp) ke = Mp;
-  double fl_acc[NC][2];
+  typedef vec AFrag[4][2];
+  auto read_a = [&](const u16* ca, AFrag& fa) {
 #pragma unroll
-  for (int c = 0; c < NC; ++c) fl_acc[c][0] = fl_acc[c][1] = 0.0;
-
-  u16x8 ah[2], al[2], uh[2], ul[2], fh[2], fl[2];
-  float4 cq[NC][4][2];  // per chain: c1, c2, alpha, beta of this thread's 8 images
-  auto load = [&](long k0) {
-    const long kk = k0 + 8 * sch;
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int m = sn + 64 * h;
-      ah[h] = *reinterpret_cast<const u16x8*>(UhT + (long)(mu0 + m) * Mp + kk);
-      al[h] = *reinterpret_cast<const u16x8*>(UlT + (long)(mu0 + m) * Mp + kk);
-      if (diag) {
-        uh[h] = ah[h];
-        ul[h] = al[h];
-      } else {
-        uh[h] = *reinterpret_cast<const u16x8*>(UhT + (long)(nu0 + m) * Mp + kk);
-        ul[h] = *reinterpret_cast<const u16x8*>(UlT + (long)(nu0 + m) * Mp + kk);
-      }
-      fh[h] = *reinterpret_cast<const u16x8*>(FhT + (long)(nu0 + m) * Mp + kk);
-      fl[h] = *reinterpret_cast<const u16x8*>(FlT + (long)(nu0 + m) * Mp + kk);
+    for (int i = 0; i < 4; ++i) {
+      const int o = frag_off(i, lane);
+      fa[i][0] = *reinterpret_cast<const vec*>(ca + o);
+      fa[i][1] = *reinterpret_cast<const vec*>(ca + kBE + o);
     }
+  };
+  // one K step t from ring position K % 4 (B slot K % 2), A fragments already
+  // in `cur`; those of step t + 1 are read into `nxt`.  Each step issues
+  // exactly 7 loads per wave (step t + 3, clamped), so vmcnt(7) at its end
+  // means everything issued before it has landed: A of t + 1 and the raw
+  // blocks and coefficients of t + 2.
+  auto step = [&](auto is_diag, auto kk, long t, AFrag& cur, AFrag& nxt) {
+    constexpr int K = decltype(kk)::value;
+    lds_barrier();   // B of step t stored; ring position (K + 3) % 4 free
+    issue(t + 3, (K + 3) & 3);
+    const u16* cb = sB[K & 1];
+    vec fb[NC][2];
+    const int ob = frag_off(wave, lane);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const float* base = coef32_all + (long)(chain0 + c) * 4 * Mp + kk;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        if (q >= 2 && !diag) continue;
-        cq[c][q][0] = *reinterpret_cast<const float4*>(base + q * Mp);
-        cq[c][q][1] = *reinterpret_cast<const float4*>(base + q * Mp + 4);
-      }
+      fb[c][0] = *reinterpret_cast<const vec*>(cb + (2 * c) * kBE + ob);
+      fb[c][1] = *reinterpret_cast<const vec*>(cb + (2 * c + 1) * kBE + ob);
     }
-  };
-  auto build_store = [&](u16* st) {
+    read_a(sA[(K + 1) & 3], nxt);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      const int m = sn + 64 * h;
-      const int o = lds_off(m, sch);
-      *reinterpret_cast<u16x8*>(st + o) = ah[h];
-      *reinterpret_cast<u16x8*>(st + kTileElems + o) = al[h];
-      float uu[8], ff[8];
-#pragma unroll
-      for (int t = 0; t < 8; ++t) {
-        uu[t] = to_f<H>(uh[h][t]) + to_f<H>(ul[h][t]);
-        ff[t] = to_f<H>(fh[h][t]) + to_f<H>(fl[h][t]);
-      }
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const float c1v[8] = {cq[c][0][0].x, cq[c][0][0].y, cq[c][0][0].z, cq[c][0][0].w,
-                              cq[c][0][1].x, cq[c][0][1].y, cq[c][0][1].z, cq[c][0][1].w};
-        const float c2v[8] = {cq[c][1][0].x, cq[c][1][0].y, cq[c][1][0].z, cq[c][1][0].w,
-                              cq[c][1][1].x, cq[c][1][1].y, cq[c][1][1].z, cq[c][1][1].w};
-        u16x8 bh, bl;
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          const float b = c1v[t] * ff[t] * gF[c][h] + c2v[t] * uu[t] * gU[c][h];
-          const u16 hb = from_f<H>(b);
-          bh[t] = hb;
-          bl[t] = from_f<H>(b - to_f<H>(hb));
-        }
-        *reinterpret_cast<u16x8*>(st + (2 + 2 * c) * kTileElems + o) = bh;
-        *reinterpret_cast<u16x8*>(st + (3 + 2 * c) * kTileElems + o) = bl;
-        if (diag) {
-          const float av[8] = {cq[c][2][0].x, cq[c][2][0].y, cq[c][2][0].z, cq[c][2][0].w,
-                               cq[c][2][1].x, cq[c][2][1].y, cq[c][2][1].z, cq[c][2][1].w};
-          const float bv[8] = {cq[c][3][0].x, cq[c][3][0].y, cq[c][3][0].z, cq[c][3][0].w,
-                               cq[c][3][1].x, cq[c][3][1].y, cq[c][3][1].z, cq[c][3][1].w};
-          float fsum = 0.f;
-#pragma unroll
-          for (int t = 0; t < 8; ++t) fsum += av[t] * ff[t] * inv_sf[h] + bv[t] * uu[t] * fyu[h];
-          fl_acc[c][h] += (double)fsum;
-        }
-      }
-    }
-  };
-  const long nk = (ke > kb) ? (ke - kb) / SO_KC : 0;
-  if (nk > 0) {
-    load(kb);
-    build_store(smem);
-  }
-  __syncthreads();
-  for (long kc = 0; kc < nk; ++kc) {
-    u16* cur = smem + (kc & 1) * kStage;
-    if (kc + 1 < nk) load(kb + (kc + 1) * SO_KC);
-    // per k-step: issue all fragment reads, then the MFMAs product-major
-#pragma unroll
-    for (int ksb = 0; ksb < 2; ++ksb) {
-      vec fa[2][2], fb[NC][2][2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const int oo = lds_off(64 * wr + 32 * i + lr, 2 * ksb + lh);
-        fa[i][0] = *reinterpret_cast<const vec*>(cur + oo);
-        fa[i][1] = *reinterpret_cast<const vec*>(cur + kTileElems + oo);
-      }
+    for (int pr = 0; pr < 3; ++pr)
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int oo = lds_off(64 * wc + 32 * j + lr, 2 * ksb + lh);
-          fb[c][j][0] = *reinterpret_cast<const vec*>(cur + (2 + 2 * c) * kTileElems + oo);
-          fb[c][j][1] = *reinterpret_cast<const vec*>(cur + (3 + 2 * c) * kTileElems + oo);
-        }
-#pragma unroll
-      for (int pr = 0; pr < 3; ++pr)
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-#pragma unroll
-          for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-              acc[c][i][j] = Mma<H>::mma(fa[i][pr == 2 ? 1 : 0], fb[c][j][pr == 1 ? 1 : 0],
-                                         acc[c][i][j]);
+        for (int i = 0; i < 4; ++i)
+          acc[c][i] = Mma<H>::mma(cur[i][pr == 2 ? 1 : 0], fb[c][pr == 1 ? 1 : 0], acc[c][i]);
+    // B of step t + 1 (clamped at the end of the range) into the other B slot
+    build(is_diag, sR[(K + 1) & 3], sC[(K + 1) & 3], sB[(K + 1) & 1], t + 1 < n ? 1.0 : 0.0);
+    TDS_WAIT_VMCNT(7);
+  };
+
+  auto run = [&](auto is_diag) {
+    // prologue: steps 0..2 requested; B of step 0 built, A fragments of
+    // step 0 read
+    issue(0, 0);
+    issue(1, 1);
+    issue(2, 2);
+    TDS_WAIT_VMCNT(0);
+    lds_barrier();
+    build(is_diag, sr0, sc0, sb0, 1.0);
+    AFrag f0, f1;
+    read_a(sa0, f0);
+    for (long t = 0; t < n; t += 4) {
+      step(is_diag, ic<0>{}, t, f0, f1);
+      step(is_diag, ic<1>{}, t + 1, f1, f0);
+      step(is_diag, ic<2>{}, t + 2, f0, f1);
+      step(is_diag, ic<3>{}, t + 3, f1, f0);
     }
-    if (kc + 1 < nk) build_store(smem + ((kc + 1) & 1) * kStage);
-    __syncthreads();
+  };
+  if (n > 0) {
+    if (diag)
+      run(std::true_type{});
+    else
+      run(std::false_type{});
   }
+  TDS_WAIT_VMCNT(0);
+  lds_barrier();
+
   // raw partial tiles (FP32, exact copies of the accumulators)
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     float* zp = zpart_all + ((long)(chain0 + c) * ksplit + ks) * np * np;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const int mm = 64 * wr + 32 * i + 8 * (r >> 2) + 4 * lh + (r & 3);
-          const int nn = 64 * wc + 32 * j + lr;
-          zp[(long)(mu0 + mm) * np + nu0 + nn] = acc[c][i][j][r];
-        }
+      for (int r = 0; r < 16; ++r) {
+        const int mm = 32 * i + 8 * (r >> 2) + 4 * lh + (r & 3);
+        const int nn = 32 * wave + lr;
+        zp[(long)(mu0 + mm) * np + nu0 + nn] = acc[c][i][r];
+      }
   }
   if (diag) {
-    double* red = reinterpret_cast<double*>(smem);  // [NC][128][4]
+    double* red = reinterpret_cast<double*>(sb0);  // [NC][2][128]
 #pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int h = 0; h < 2; ++h) red[(c * SO_T + sn + 64 * h) * 4 + sch] = fl_acc[c][h];
-    __syncthreads();
-    for (int e = tid; e < NC * SO_T; e += blockDim.x) {
-      const int c = e / SO_T, nn = e % SO_T;
-      const double* q = red + (c * SO_T + nn) * 4;
-      fpart_all[((long)(chain0 + c) * ksplit + ks) * np + nu0 + nn] = (q[0] + q[1]) + (q[2] + q[3]);
+    for (int c = 0; c < NC; ++c) red[(c * 2 + (tid >> 7)) * kBR + brow] = fl_acc[c];
+    lds_barrier();
+    for (int e = tid; e < NC * kBR; e += 256) {
+      const int c = e / kBR, nn = e % kBR;
+      fpart_all[((long)(chain0 + c) * ksplit + ks) * np + nu0 + nn] =
+          red[(c * 2 + 0) * kBR + nn] + red[(c * 2 + 1) * kBR + nn];
     }
   }
 }
@@ -668,11 +859,34 @@ __global__ void reduce_ksplit_s32(const float* __restrict__ zpart_all,
 }
 
 // ---------------------------------------------------------------- host side
+// K split of the outer GEMM: ntile^2 * ks work-groups (ks a multiple of 8, one
+// range per XCD in turn) filling whole waves of 256 CUs as closely as
+// possible, the smallest such ks, with at least 32 K steps per range.
+static int choose_ksplit_split(int ntile, long Mp) {
+  const long nsub = Mp / kBK;   // ranges are whole units of 4 steps
+  const int t2 = ntile * ntile;
+  int best = 8;
+  double best_eff = -1.0;
+  for (int ks = 8; ks <= 512; ks += 8) {
+    if (ks > 8 && nsub / ks < 32) break;
+    const long wgs = (long)t2 * ks;
+    const long rounds = (wgs + 255) / 256;
+    const double eff = (double)wgs / (double)(rounds * 256);
+    if (eff > best_eff + 0.02) {
+      best_eff = eff;
+      best = ks;
+    }
+  }
+  return best;
+}
+
 int ensure_split(Ctx* c, int prec) {
+  static_assert(kMaxChains == 8, "coefficient block layout assumes 8 chains");
   SplitStore& st = c->split[prec];
   if (st.ready) return 0;
   const long Mp = c->Mp;
   const int np = c->np;
+  TDS_REQUIRE(Mp % kBR == 0 && np % kBR == 0, 3, "split path: padded sizes must be multiples of 128");
   const size_t half_bytes = sizeof(u16) * (size_t)Mp * np;
   void** arr[] = {&st.Uh, &st.Ul, &st.Fh, &st.Fl, &st.UhT, &st.UlT, &st.FhT, &st.FlT};
   for (void** p : arr) TDS_CHECK_HIP(hipMalloc(p, half_bytes));
@@ -680,21 +894,17 @@ int ensure_split(Ctx* c, int prec) {
   TDS_CHECK_HIP(hipMalloc((void**)&st.sf, sizeof(double) * np));
   TDS_CHECK_HIP(hipMalloc((void**)&st.umax, sizeof(double) * np));
   TDS_CHECK_HIP(hipMalloc((void**)&st.fmax, sizeof(double) * np));
+  TDS_CHECK_HIP(hipMalloc((void**)&st.colw, sizeof(float) * 3 * np));
   if (!c->Y1h) {
     TDS_CHECK_HIP(hipMalloc(&c->Y1h, sizeof(u16) * kMaxChains * (size_t)np * np));
     TDS_CHECK_HIP(hipMalloc(&c->Y1l, sizeof(u16) * kMaxChains * (size_t)np * np));
-    TDS_CHECK_HIP(hipMalloc((void**)&c->epi, sizeof(double) * kMaxChains * np * 4));
+    TDS_CHECK_HIP(hipMalloc((void**)&c->epi, sizeof(float) * kMaxChains * 2 * np));
+    TDS_CHECK_HIP(hipMalloc((void**)&c->ysc, sizeof(double) * kMaxChains));
     TDS_CHECK_HIP(hipMalloc((void**)&c->cmax, sizeof(double) * kMaxChains * 2));
     TDS_CHECK_HIP(hipMalloc((void**)&c->coef32, sizeof(float) * kMaxChains * Mp * 4));
-    TDS_CHECK_HIP(hipMalloc((void**)&c->wpart, sizeof(double) * kMaxChains * (np / SW_BN) * Mp * 4));
-    // K split of the split paths: ~1536 work-groups over the 128 x 128 tiles,
-    // at least 2048 images per range, multiple of 8 (one range per XCD in turn)
-    const int ntile = np / SO_T;
-    long ks = (1536 + ntile * ntile - 1) / (ntile * ntile);
-    ks = (ks + 7) / 8 * 8;
-    while (ks > 8 && ks * 2048 > Mp) ks -= 8;
-    if (ks < 8) ks = 8;
-    c->ksplit_s = (int)ks;
+    TDS_CHECK_HIP(hipMalloc((void**)&c->wpart, sizeof(double) * kMaxChains * (np / kBR) * Mp * 4));
+    const int ks = choose_ksplit_split(np / kBR, Mp);
+    c->ksplit_s = ks;
     TDS_CHECK_HIP(hipMalloc((void**)&c->zpart32, sizeof(float) * kMaxChains * (size_t)ks * np * np));
     TDS_CHECK_HIP(hipMalloc((void**)&c->fpart_s, sizeof(double) * kMaxChains * (size_t)ks * np));
     TDS_CHECK_HIP(hipMalloc((void**)&c->sbsc, sizeof(double) * kMaxChains * np));
@@ -706,6 +916,8 @@ int ensure_split(Ctx* c, int prec) {
                      np, st.umax, st.fmax);
   hipLaunchKernelGGL(scales_kernel, dim3((np + 127) / 128), dim3(128), 0, s, st.umax, st.fmax, np,
                      st.su, st.sf);
+  hipLaunchKernelGGL(colw_kernel, dim3((np + 127) / 128), dim3(128), 0, s, st.su, st.sf, c->fpsi,
+                     np, st.colw);
   dim3 g((unsigned)(Mp / 32), np / 32);
   if (prec == PREC_FP16X2)
     hipLaunchKernelGGL(split_images<_Float16>, g, dim3(256), 0, s, c->U, c->F, Mp, np, st.su,
@@ -722,27 +934,26 @@ int ensure_split(Ctx* c, int prec) {
 }
 
 template <typename H, int NC>
-static void launch_group(Ctx* c, const SplitStore& st, int chain0, int apply_d4, hipStream_t s) {
+static void launch_weights(Ctx* c, const SplitStore& st, int chain0, hipStream_t s) {
   const int np = c->np;
-  const long Mp = c->Mp;
-  const int nct = np / SW_BN;
-  const int nrt = (int)(Mp / SW_BM);
-  const int nrt8 = round_up(nrt, 8);
-  hipLaunchKernelGGL((weights_split<H, NC>), dim3((unsigned)(nrt8 * nct)), dim3(256), 0, s,
+  const int nct = np / kBR;
+  const int nrt = (int)(c->Mp / kBR);
+  // persistent: at most 32 work-groups per XCD, none without a tile
+  const int per_xcd_tiles = (nrt + 7) / 8 * nct;
+  const int nper = per_xcd_tiles < 32 ? per_xcd_tiles : 32;
+  hipLaunchKernelGGL((weights_split<H, NC>), dim3((unsigned)(8 * nper)), dim3(256), 0, s,
                      (const u16*)st.Uh, (const u16*)st.Ul, (const u16*)st.Fh, (const u16*)st.Fl,
-                     (const u16*)c->Y1h, (const u16*)c->Y1l, c->epi, np, Mp, chain0, nrt,
-                     c->wpart);
-  (void)apply_d4;
+                     (const u16*)c->Y1h, (const u16*)c->Y1l, st.colw, c->epi, np, c->Mp, chain0,
+                     nrt, c->wpart);
 }
 
 template <typename H, int NC>
-static void launch_outer(Ctx* c, const SplitStore& st, int chain0, long rps, hipStream_t s) {
-  const int ntile = c->np / SO_T;
+static void launch_outer(Ctx* c, const SplitStore& st, int chain0, hipStream_t s) {
+  const int ntile = c->np / kBR;
   hipLaunchKernelGGL((outer_split<H, NC>), dim3((unsigned)(ntile * ntile * c->ksplit_s)),
                      dim3(256), 0, s, (const u16*)st.UhT, (const u16*)st.UlT, (const u16*)st.FhT,
-                     (const u16*)st.FlT, c->coef32, c->cmax, st.umax, st.fmax, st.su, st.sf,
-                     c->fy, c->np, c->Mp, c->ksplit_s, ntile, rps, chain0, c->zpart32,
-                     c->fpart_s, c->sbsc);
+                     (const u16*)st.FlT, c->coef32, st.su, st.sf, c->fy, c->np, c->Mp,
+                     c->ksplit_s, ntile, chain0, c->zpart32, c->fpart_s);
 }
 
 template <typename H>
@@ -751,30 +962,34 @@ static int launch_all(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* e
   const int np = c->np;
   const long Mp = c->Mp;
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(prep_y<H>, dim3(np, nc), dim3(256), 0, s, c->Y1, c->R1, st.su, st.sf, c->fy,
-                     c->fpsi, np, (u16*)c->Y1h, (u16*)c->Y1l, c->epi, c->cmax);
-  // chains in groups of up to 3 per pass (two LDS stages of (2 + 2 NC) tiles)
+  hipLaunchKernelGGL(ymax_kernel, dim3(nc), dim3(256), 0, s, c->Y1, st.su, np, c->ysc);
+  hipLaunchKernelGGL(prep_y<H>, dim3(np, nc), dim3(256), 0, s, c->Y1, c->R1, st.su, c->fy, np,
+                     c->ysc, (u16*)c->Y1h, (u16*)c->Y1l, c->epi, c->cmax);
+  // chains in groups of up to 3 per pass
   for (int c0 = 0; c0 < nc;) {
     const int g = nc - c0 >= 3 ? 3 : nc - c0;
     switch (g) {
-      case 3: launch_group<H, 3>(c, st, c0, apply_d4, s); break;
-      case 2: launch_group<H, 2>(c, st, c0, apply_d4, s); break;
-      default: launch_group<H, 1>(c, st, c0, apply_d4, s); break;
+      case 3: launch_weights<H, 3>(c, st, c0, s); break;
+      case 2: launch_weights<H, 2>(c, st, c0, s); break;
+      default: launch_weights<H, 1>(c, st, c0, s); break;
     }
     c0 += g;
   }
-  const int nct = np / SW_BN;
+  const int nct = np / kBR;
   hipLaunchKernelGGL(finalize_coef, dim3((unsigned)((Mp + 255) / 256), nc), dim3(256), 0, s,
-                     c->wpart, c->rho_img, Mp, nct, apply_d4, (double*)nullptr, c->coef32,
+                     c->wpart, c->rho_img, c->ysc, Mp, nct, apply_d4, (double*)nullptr, c->coef32,
                      c->cmax);
+  hipLaunchKernelGGL(bscale_kernel, dim3(nc), dim3(256), 0, s, c->cmax, st.umax, st.fmax, c->fy, np,
+                     c->sbsc);
+  hipLaunchKernelGGL(scale_coef_kernel, dim3((unsigned)((Mp + 255) / 256), nc), dim3(256), 0, s,
+                     c->sbsc, np, Mp / kBK, c->coef32);
   if (ev) (void)hipEventRecord(ev[1], s);
-  const long rps = round_up_l((Mp + c->ksplit_s - 1) / c->ksplit_s, SO_KC);
   for (int c0 = 0; c0 < nc;) {
     const int g = nc - c0 >= 3 ? 3 : nc - c0;
     switch (g) {
-      case 3: launch_outer<H, 3>(c, st, c0, rps, s); break;
-      case 2: launch_outer<H, 2>(c, st, c0, rps, s); break;
-      default: launch_outer<H, 1>(c, st, c0, rps, s); break;
+      case 3: launch_outer<H, 3>(c, st, c0, s); break;
+      case 2: launch_outer<H, 2>(c, st, c0, s); break;
+      default: launch_outer<H, 1>(c, st, c0, s); break;
     }
     c0 += g;
   }
