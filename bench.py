@@ -53,6 +53,29 @@ def spectrum(a, b, w_grid, eta):
     return -np.imag(gf)
 
 
+def copy_probe(nbytes):
+    """One device-to-device copy of a known size through the library, for
+    scaling the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    from tdscha_amd import _lib
+    a = _lib.DeviceBuffer((nbytes,), "uint8")
+    b = _lib.DeviceBuffer((nbytes,), "uint8")
+    _lib.check(_lib.lib().tdscha_memcpy(b.ptr, a.ptr, nbytes, 3), "copy")   # synchronous
+    a.free()
+    b.free()
+
+
+def profiled_traffic(kernel, precision):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%s_128atoms_20k.txt" % (kernel, precision))
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def cpu_baseline(ens, lanc, n_sample_cfg, budget_s):
     """Reference C kernels (oracle/_ref, compiled from LanczosFunctions.c) on a
     bounded sample of the same workload, single thread; falls back to the
@@ -101,6 +124,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="bytes of one device-to-device copy to run first (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the ensemble sum "
@@ -110,6 +135,8 @@ def main():
 
     from tdscha_amd import _lib
     from tdscha_amd.lanczos import Lanczos
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
 
     ens = build_workload(args.supercell, args.configs, 300.0, seed=0)
     lanc = Lanczos(ens)
@@ -182,7 +209,8 @@ def main():
     achieved = alg * PRODUCTS[args.precision] / (pm[dom] * 1e-3)
     roof = {"bound": "mfma", "kernel": kname, "achieved": achieved / 1e12,
             "peak": PEAK[args.precision] / 1e12, "unit": "TFLOP/s",
-            "frac": achieved / PEAK[args.precision], "traffic": None}
+            "frac": achieved / PEAK[args.precision],
+            "traffic": profiled_traffic(kname, args.precision)}
 
     cpu = None
     if not args.no_cpu:
