@@ -135,9 +135,12 @@ __global__ void finalize_pa(const double* __restrict__ zsum_all, const double* _
       norm * (fy[mu] * z[(long)mu * np + nu] + fy[nu] * z[(long)nu * np + mu]);
 }
 
-// ---------------------------------------------------------------- small GEMM
-// C[b] = op(A[b]) * op(B[b]); row-major; 32x32 tiles, 256 threads, 2x2 per
-// thread, K in steps of 32 through LDS (FP64).
+// Batched FP64 GEMM C = op(A) op(B) on MFMA (v_mfma_f64_16x16x4f64): 64 x 64
+// tiles, 4 waves of 32 x 32 (2 x 2 blocks of 16 x 16), K in chunks of 16
+// through LDS.  ta / tb select A^T / B^T; the LDS fill is coalesced along the
+// contiguous index of each operand.  Operand layout of the 16x16x4 FP64 MFMA:
+// A[i][k]: i = lane % 16, k = lane / 16; B likewise with j; result
+// D[i][j]: i = 4 r + lane / 16, j = lane % 16 (r = 0..3).
 __global__ __launch_bounds__(256) void gemm_f64(int M, int N, int K, const double* __restrict__ A,
                                                 long lda, long sA, int ta,
                                                 const double* __restrict__ B, long ldb, long sB,
@@ -146,49 +149,63 @@ __global__ __launch_bounds__(256) void gemm_f64(int M, int N, int K, const doubl
   A += b * sA;
   B += b * sB;
   C += b * sC;
-  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  __shared__ double As[32][33];
-  __shared__ double Bs[32][33];
-  double acc[2][2] = {};
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    for (int e = threadIdx.x; e < 32 * 32; e += 256) {
-      const int r = e >> 5, q = e & 31;
-      // A tile element (m0 + mm, k0 + kk); coalesced along the contiguous index
-      int mm = ta ? q : r, kk = ta ? r : q;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  __shared__ double As[16][65];
+  __shared__ double Bs[16][65];
+  v4d acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0, 0, 0, 0};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int e = tid; e < 16 * 64; e += 256) {
+      // A tile element (m0 + mm, k0 + kk)
+      int mm, kk;
+      if (ta) { kk = e >> 6; mm = e & 63; } else { mm = e >> 4; kk = e & 15; }
       int gm = m0 + mm, gk = k0 + kk;
       As[kk][mm] = (gm < M && gk < K) ? (ta ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk]) : 0.0;
-      int nn = tb ? r : q;
-      kk = tb ? q : r;
+      // B tile element (k0 + kk, n0 + nn)
+      int nn;
+      if (tb) { nn = e >> 4; kk = e & 15; } else { kk = e >> 6; nn = e & 63; }
       const int gn = n0 + nn;
       gk = k0 + kk;
       Bs[kk][nn] = (gn < N && gk < K) ? (tb ? B[(long)gn * ldb + gk] : B[(long)gk * ldb + gn]) : 0.0;
     }
     __syncthreads();
-#pragma unroll 8
-    for (int kk = 0; kk < 32; ++kk) {
-      const double a0 = As[kk][ty], a1 = As[kk][ty + 16];
-      const double b0 = Bs[kk][tx], b1 = Bs[kk][tx + 16];
-      acc[0][0] += a0 * b0;
-      acc[0][1] += a0 * b1;
-      acc[1][0] += a1 * b0;
-      acc[1][1] += a1 * b1;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int k = 4 * ks + (lane >> 4);
+      double a[2], bb[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[k][32 * wr + 16 * i + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bb[j] = Bs[k][32 * wc + 16 * j + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bb[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int gm = m0 + ty + 16 * i, gn = n0 + tx + 16 * j;
-      if (gm < M && gn < N) C[(long)gm * ldc + gn] = acc[i][j];
-    }
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gm = m0 + 32 * wr + 16 * i + 4 * r + (lane >> 4);
+        const int gn = n0 + 32 * wc + 16 * j + (lane & 15);
+        if (gm < M && gn < N) C[(long)gm * ldc + gn] = acc[i][j][r];
+      }
 }
 
 static void gemm(int batch, int M, int N, int K, const double* A, long lda, long sA, int ta,
                  const double* B, long ldb, long sB, int tb, double* C, long ldc, long sC,
                  hipStream_t s) {
-  dim3 g((N + 31) / 32, (M + 31) / 32, batch);
+  dim3 g((N + 63) / 64, (M + 63) / 64, batch);
   hipLaunchKernelGGL(gemm_f64, g, dim3(256), 0, s, M, N, K, A, lda, sA, ta, B, ldb, sB, tb, C, ldc,
                      sC);
 }
@@ -212,15 +229,27 @@ __global__ void perm_average(const double* __restrict__ Ccart, const int* __rest
 
 // f <- P_T f  (P_T = mean of the mode-space translation operators); tprojT
 // holds P_T transposed so that consecutive threads read consecutive entries
-__global__ void project_f(const double* __restrict__ tprojT, const double* __restrict__ f_all,
-                          int n, double* __restrict__ out_all) {
+// out = P_T f: a block of 32 output modes, the K = n sum split over 8 thread
+// rows and combined in LDS (fixed order)
+__global__ __launch_bounds__(256) void project_f(const double* __restrict__ tprojT,
+                                                 const double* __restrict__ f_all, int n,
+                                                 double* __restrict__ out_all) {
   const int chain = blockIdx.y;
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int m = blockIdx.x * 32 + tx;
   const double* f = f_all + (long)chain * n;
   double s = 0.0;
-  for (int k = 0; k < n; ++k) s += tprojT[(long)k * n + m] * f[k];
-  out_all[(long)chain * n + m] = s;
+  if (m < n)
+    for (int k = ty; k < n; k += 8) s += tprojT[(long)k * n + m] * f[k];
+  __shared__ double red[8][32];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && m < n) {
+    double t = 0.0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) t += red[r][tx];
+    out_all[(long)chain * n + m] = t;
+  }
 }
 
 int project_translations(Ctx* c, int nc, hipStream_t s) {
@@ -240,7 +269,7 @@ int project_translations(Ctx* c, int nc, hipStream_t s) {
   // d2v = mode_tmp * pols            (n x n)
   gemm(nc, n, n, nc_, c->mode_tmp, nc_, (long)n * nc_, 0, c->pols, n, 0, 0, c->d2v_fin, n,
        (long)n * n, s);
-  hipLaunchKernelGGL(project_f, dim3((n + 127) / 128, nc), dim3(128), 0, s, c->tproj, c->f_fin, n,
+  hipLaunchKernelGGL(project_f, dim3((n + 31) / 32, nc), dim3(256), 0, s, c->tproj, c->f_fin, n,
                      c->f_proj);
   return 0;
 }

@@ -35,6 +35,7 @@ constexpr double RY_TO_K = 157887.32400374097;
 constexpr int kTile = 128;      // column tile of the weighted outer-product GEMM
 constexpr int kRowTile = 64;    // image rows per workgroup of the weights GEMM
 constexpr int kMaxChains = 8;   // perturbations applied in one batched call
+constexpr int kPartBlocks = 1024;  // blocks of the grid-wide maxima in the split prep
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 inline long round_up_l(long x, long m) { return (x + m - 1) / m * m; }
@@ -101,6 +102,7 @@ struct Ctx {
   double* ysc = nullptr;    // [kMaxChains] power-of-two scale of split Y1
   double* bsc = nullptr;    // [kMaxChains, np] column scale of split B
   double* cmax = nullptr;   // [kMaxChains, 2] max|c1|, max|c2| (device)
+  double* pmax = nullptr;   // [kMaxChains, kPartBlocks, 2] per-block maxima of the split prep
   float* epi = nullptr;     // [kMaxChains, 2, np] fY R1 and R1 (weights epilogue)
   float* coef32 = nullptr;  // [Mp/16, kMaxChains, 4, 16] FP32 coefficients for the outer GEMM
   double* wpart = nullptr;  // [kMaxChains, np/128, Mp, 4] weights row-dot partials

@@ -255,7 +255,7 @@ void tdscha_ctx_destroy(tdscha_ctx* ctx) {
                   c->pair_row, c->pair_col, c->std_coef, c->yw, c->reaw, c->pols, c->inv_perms,
                   c->tproj, c->cart_tmp, c->cart_tmp2, c->mode_tmp, c->psi_dev, c->psi_out_dev,
                   c->stage_in, c->stage_out, c->f_fin, c->d2v_fin, c->f_proj, c->epi,
-                  c->coef32, c->wpart, c->zpart32, c->fpart_s, c->sbsc};
+                  c->coef32, c->wpart, c->zpart32, c->fpart_s, c->sbsc, c->pmax};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& s : c->split) {

@@ -201,18 +201,36 @@ __global__ void split_images(const double* __restrict__ U, const double* __restr
 
 // ------------------------------------------------------------ per-call prep
 // One power-of-two scale per chain for the split Y1: sy with
-// max_jk |Y1[j][k] / su[k]| * sy < 2^12.  One block per chain.
-__global__ void ymax_kernel(const double* __restrict__ Y1all, const double* __restrict__ su,
-                            int np, double* __restrict__ ysc) {
-  const int chain = blockIdx.x;
+// max_jk |Y1[j][k] / su[k]| * sy < 2^12.  Block maxima over slices of Y1
+// (grid kYmaxBlocks x chains), then one block per chain.
+constexpr int kYmaxBlocks = 64;
+__global__ void ymax_part_kernel(const double* __restrict__ Y1all, const double* __restrict__ su,
+                                 int np, double* __restrict__ pmax) {
+  const int chain = blockIdx.y;
   const double* Y = Y1all + (long)chain * np * np;
+  const long nn = (long)np * np, per = (nn + gridDim.x - 1) / gridDim.x;
+  const long lo = blockIdx.x * per, hi = lo + per < nn ? lo + per : nn;
   __shared__ double red[256];
   double mx = 0.0;
-  for (long e = threadIdx.x; e < (long)np * np; e += blockDim.x)
-    mx = fmax(mx, fabs(Y[e] / su[e % np]));
+  for (long e = lo + threadIdx.x; e < hi; e += blockDim.x) mx = fmax(mx, fabs(Y[e] / su[e % np]));
   red[threadIdx.x] = mx;
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) pmax[((long)chain * kPartBlocks + blockIdx.x) * 2] = red[0];
+}
+
+__global__ void ymax_kernel(const double* __restrict__ pmax, int nblocks, double* __restrict__ ysc) {
+  const int chain = blockIdx.x;
+  __shared__ double red[64];
+  double mx = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
+    mx = fmax(mx, pmax[((long)chain * kPartBlocks + b) * 2]);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
     if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
     __syncthreads();
   }
@@ -220,14 +238,13 @@ __global__ void ymax_kernel(const double* __restrict__ Y1all, const double* __re
 }
 
 // YB[c] block (column tile, K step) holds Y1[c][j][k] sy / su[k] split hi/lo;
-// per-chain epilogue column factors (fY R1, R1) in FP32; resets the |c|
-// maxima.  One block per (column j, chain).
+// per-chain epilogue column factors (fY R1, R1) in FP32.  One block per
+// (column j, chain).
 template <typename H>
 __global__ void prep_y(const double* __restrict__ Y1all, const double* __restrict__ R1all,
                        const double* __restrict__ su, const double* __restrict__ fy, int np,
                        const double* __restrict__ ysc, u16* __restrict__ YBh,
-                       u16* __restrict__ YBl, float* __restrict__ epw,
-                       double* __restrict__ cmax) {
+                       u16* __restrict__ YBl, float* __restrict__ epw) {
   const int j = blockIdx.x, chain = blockIdx.y;
   const double* Y = Y1all + ((long)chain * np + j) * np;
   const double sy = ysc[chain];
@@ -244,10 +261,6 @@ __global__ void prep_y(const double* __restrict__ Y1all, const double* __restric
     const double R = R1all[(long)chain * np + j];
     epw[((long)chain * 2 + 0) * np + j] = (float)(fy[j] * R);
     epw[((long)chain * 2 + 1) * np + j] = (float)R;
-    if (j == 0) {
-      cmax[chain * 2 + 0] = 0.0;
-      cmax[chain * 2 + 1] = 0.0;
-    }
   }
 }
 
@@ -513,18 +526,17 @@ __global__ __launch_bounds__(256, 1) void weights_split(
 __global__ void finalize_coef(const double* __restrict__ part, const double* __restrict__ rho_img,
                               const double* __restrict__ ysc, long Mp, int nct, int apply_d4,
                               double* __restrict__ coef_all, float* __restrict__ coefB,
-                              double* __restrict__ cmax) {
+                              double* __restrict__ pmax) {
   const int chain = blockIdx.y;
-  const long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double isy = 1.0 / ysc[chain];   // the V-weighted sums carry the Y1 scale
   double m1 = 0.0, m2 = 0.0;
-  if (row < Mp) {
+  for (long row = (long)blockIdx.x * blockDim.x + threadIdx.x; row < Mp;
+       row += (long)gridDim.x * blockDim.x) {
     double s[4] = {0, 0, 0, 0};
     for (int ct = 0; ct < nct; ++ct) {
       const double* p = part + (((long)chain * nct + ct) * Mp + row) * 4;
       s[0] += p[0]; s[1] += p[1]; s[2] += p[2]; s[3] += p[3];
     }
-    // the V-weighted sums carry the Y1 scale
-    const double isy = 1.0 / ysc[chain];
     s[0] *= isy;
     s[1] *= isy;
     const double rho = rho_img[row];
@@ -537,35 +549,59 @@ __global__ void finalize_coef(const double* __restrict__ part, const double* __r
     }
     float* cb = coefB + (((row / kBK) * kMaxChains + chain) * 4) * kBK + row % kBK;
     cb[0] = (float)c1; cb[kBK] = (float)c2; cb[2 * kBK] = (float)al; cb[3 * kBK] = (float)be;
-    m1 = fabs(c1);
-    m2 = fabs(c2);
+    m1 = fmax(m1, fabs(c1));
+    m2 = fmax(m2, fabs(c2));
   }
   __shared__ double r1[256], r2[256];
   r1[threadIdx.x] = m1;
   r2[threadIdx.x] = m2;
-  lds_barrier();
+  __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
     if (threadIdx.x < o) {
       r1[threadIdx.x] = fmax(r1[threadIdx.x], r1[threadIdx.x + o]);
       r2[threadIdx.x] = fmax(r2[threadIdx.x], r2[threadIdx.x + o]);
     }
-    lds_barrier();
+    __syncthreads();
   }
   if (threadIdx.x == 0) {
-    atomic_max_pos(&cmax[chain * 2 + 0], r1[0]);
-    atomic_max_pos(&cmax[chain * 2 + 1], r2[0]);
+    double* pm = pmax + ((long)chain * kPartBlocks + blockIdx.x) * 2;
+    pm[0] = r1[0];
+    pm[1] = r2[0];
   }
 }
 
 // One power-of-two B scale per chain: sb with max_nu (max|c1| fmx[nu] +
 // max|c2| fY[nu] umx[nu] / 2) * sb < 2^12, written for every nu (the
 // reduction undoes it per column).  One block per chain.
-__global__ void bscale_kernel(const double* __restrict__ cmax, const double* __restrict__ umx,
-                              const double* __restrict__ fmx, const double* __restrict__ fy,
-                              int np, double* __restrict__ sb_all) {
+__global__ void bscale_kernel(const double* __restrict__ pmax, int nblocks,
+                              const double* __restrict__ umx, const double* __restrict__ fmx,
+                              const double* __restrict__ fy, int np, double* __restrict__ cmax,
+                              double* __restrict__ sb_all) {
   const int chain = blockIdx.x;
-  __shared__ double red[256];
-  const double a = cmax[chain * 2 + 0], b = 0.5 * cmax[chain * 2 + 1];
+  __shared__ double red[256], red2[256];
+  // max |c1|, max |c2| over the coefficient blocks
+  double m1 = 0.0, m2 = 0.0;
+  for (int bk = threadIdx.x; bk < nblocks; bk += blockDim.x) {
+    const double* pm = pmax + ((long)chain * kPartBlocks + bk) * 2;
+    m1 = fmax(m1, pm[0]);
+    m2 = fmax(m2, pm[1]);
+  }
+  red[threadIdx.x] = m1;
+  red2[threadIdx.x] = m2;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+      red2[threadIdx.x] = fmax(red2[threadIdx.x], red2[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  const double a = red[0], b = 0.5 * red2[0];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    cmax[chain * 2 + 0] = a;
+    cmax[chain * 2 + 1] = red2[0];
+  }
   double mx = 0.0;
   for (int nu = threadIdx.x; nu < np; nu += blockDim.x)
     mx = fmax(mx, a * fmx[nu] + b * fy[nu] * umx[nu]);
@@ -594,27 +630,29 @@ __global__ void scale_coef_kernel(const double* __restrict__ sb_all, int np, lon
 // ------------------------------------------------------------- outer GEMM
 // Z_c[mu, nu] = sum_k U[k, mu] B_c[k, nu], B_c = c1 F + (c2/2) fY U, over a
 // range of K steps (16 images each, a multiple of 4 steps per range); tile
-// 128 (mu) x 128 (nu).  The A blocks (U^T of the mu tile) and the raw U^T /
-// F^T blocks of the nu tile arrive by direct-to-LDS loads three steps ahead
-// in rings of 4 slots (separate arrays, slot index fixed at compile time by a
-// 4-fold unroll).  B_c of step t + 1 is built in FP32 from the raw blocks
-// while the MFMAs of step t run (each wave builds one half of the block, 8
-// images, so its coefficients are broadcast reads), split hi/lo again and stored to
-// the other of two B slots.  Wave w owns the 32 columns nu0 + 32 w .. for all
-// 128 mu (4 x 1 blocks of 32 x 32 per chain).  The raw FP32 accumulators go
-// to an FP32 partial buffer (exact); the scales are undone in the FP64
-// reduction.
+// 128 (mu) x 128 (nu).  The A blocks (U^T of the mu tile), the raw U^T / F^T
+// blocks of the nu tile and the step's coefficients arrive by direct-to-LDS
+// loads three steps ahead in rings of 4 slots (separate arrays, slot index
+// fixed at compile time by a 4-fold unroll).  B_c of step t + 1 is built in
+// FP32 from the raw blocks while the MFMAs of step t run, split hi/lo again
+// and stored to the other of two B slots.  8 waves, two per SIMD, so one
+// wave's build overlaps the other's MFMAs: wave (wr, wc) owns mu 64 wr ..
+// 64 wr + 63 and nu 32 wc .. 32 wc + 31 (2 x 1 blocks of 32 x 32 per chain).
+// The raw FP32 accumulators go to an FP32 partial buffer (exact); the scales
+// are undone in the FP64 reduction.
 template <typename H, int NC>
-__global__ __launch_bounds__(256, 1) void outer_split(
+__global__ __launch_bounds__(512, 1) void outer_split(
     const u16* __restrict__ UTh, const u16* __restrict__ UTl, const u16* __restrict__ FTh,
     const u16* __restrict__ FTl, const float* __restrict__ coefB,
     const double* __restrict__ su, const double* __restrict__ sf, const double* __restrict__ fy,
     int np, long Mp, int ksplit, int ntile, int chain0, float* __restrict__ zpart_all,
     double* __restrict__ fpart_all) {
   typedef typename Mma<H>::vec vec;
+  typedef u16 u16x4v __attribute__((ext_vector_type(4)));
   constexpr int kA = 2 * kBE;                   // A hi/lo
   constexpr int kR = 4 * kBE;                   // raw U^T h/l, F^T h/l
   constexpr int kB = 2 * NC * kBE;              // B hi/lo per chain
+  constexpr int kCoef = NC * 4 * kBK;           // floats of one step's coefficients
   __shared__ __align__(16) u16 sa0[kA];
   __shared__ __align__(16) u16 sa1[kA];
   __shared__ __align__(16) u16 sa2[kA];
@@ -625,7 +663,6 @@ __global__ __launch_bounds__(256, 1) void outer_split(
   __shared__ __align__(16) u16 sr3[kR];
   __shared__ __align__(16) u16 sb0[kB];
   __shared__ __align__(16) u16 sb1[kB];
-  constexpr int kCoef = NC * 4 * kBK;           // floats of one step's coefficients
   __shared__ __align__(16) float sc0[kCoef];
   __shared__ __align__(16) float sc1[kCoef];
   __shared__ __align__(16) float sc2[kCoef];
@@ -646,156 +683,150 @@ __global__ __launch_bounds__(256, 1) void outer_split(
   const bool diag = (tm == tn);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 31, lh = lane >> 5;
+  const int wr = wave >> 2, wc = wave & 3;
 
   // K steps [g0, g0 + n) in units of 4 steps
   const long nsub = Mp / kBK, nunit = nsub / 4;
   const long g0 = 4 * ((long)ks * nunit / ksplit);
   const long n = 4 * ((long)(ks + 1) * nunit / ksplit) - g0;
 
-  // builder role: thread t owns the 16 bytes at 16 t of every B block, i.e.
-  // row nu0 + t % 128, images 8 bq .. 8 bq + 7 with bq = t / 128 (one value
-  // per wave)
+  // builder role: thread t owns images 4 bq .. 4 bq + 3 (bq = t / 128, one
+  // value per wave) of row nu0 + t % 128 of every B block: 8 bytes at
+  // element (bq / 2) * 1024 + row * 8 + 4 (bq % 2) of the chunk-major block
   const int brow = tid & (kBR - 1);
   const int bq = tid >> 7;
+  const int boff = (bq >> 1) * (kBR * 8) + brow * 8 + 4 * (bq & 1);
   const int nu = nu0 + brow;
   const float inv_sf = (float)(1.0 / sf[nu]);
   const float hfu = (float)(0.5 * fy[nu] / su[nu]);
+  const float fyu = (float)(fy[nu] / su[nu]);
 
   const long a_tile = (long)tm * nsub * kBE;
   const long b_tile = (long)tn * nsub * kBE;
-  const long src = (long)wave * 512 + lane * 8;
-  // A and raw blocks and coefficients of step ts (clamped to the range) into
-  // ring position p: 7 loads per wave
+  // the 24 1 KB pieces of a step's A and raw blocks, 3 per wave, and a
+  // 32 NC-byte share of its coefficients: 4 loads per wave
   auto issue = [&](long ts, int p) {
     if (ts > n - 1) ts = n - 1;
-    const long offa = a_tile + (g0 + ts) * kBE + src;
-    const long offb = b_tile + (g0 + ts) * kBE + src;
-    u16* a = sA[p] + wave * 512;
-    u16* r = sR[p] + wave * 512;
-    dma16(UTh + offa, a);
-    dma16(UTl + offa, a + kBE);
-    dma16(UTh + offb, r);
-    dma16(UTl + offb, r + kBE);
-    dma16(FTh + offb, r + 2 * kBE);
-    dma16(FTl + offb, r + 3 * kBE);
-    // NC * 256 bytes of coefficients, a quarter per wave: lanes 0 .. 4 NC - 1
-    const float* cs = coefB + ((g0 + ts) * kMaxChains + chain0) * 4 * kBK + wave * NC * 16 + lane * 4;
-    if (lane < 4 * NC) dma16(cs, sC[p] + wave * NC * 16);
+    const long offa = a_tile + (g0 + ts) * kBE;
+    const long offb = b_tile + (g0 + ts) * kBE;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int q = wave + 8 * m;               // piece: block q / 4, part q % 4
+      const int blk = q >> 2, part = (q & 3) * 512;
+      const u16* src = blk == 0 ? UTh + offa : blk == 1 ? UTl + offa : blk == 2 ? UTh + offb
+                     : blk == 3 ? UTl + offb : blk == 4 ? FTh + offb : FTl + offb;
+      u16* dst = blk < 2 ? sA[p] + blk * kBE : sR[p] + (blk - 2) * kBE;
+      dma16(src + part + lane * 8, dst + part);
+    }
+    const float* cs = coefB + ((g0 + ts) * kMaxChains + chain0) * 4 * kBK + wave * NC * 8 + lane * 4;
+    if (lane < 2 * NC) dma16(cs, sC[p] + wave * NC * 8);
   };
 
   double fl_acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) fl_acc[c] = 0.0;
-  // B of step ts (clamped) from its raw blocks; on diagonal tiles also the f
-  // partial sums (weighted by `live`, 0 for a clamped step)
+  // B of one step from its raw blocks and coefficients; on diagonal tiles
+  // also the f partial sums (weighted by `live`, 0 for a clamped step)
   auto build = [&](auto is_diag, const u16* rs, const float* cs, u16* b_slot, double live) {
     constexpr bool DIAG = decltype(is_diag)::value;
-    const u16x8 r0 = *reinterpret_cast<const u16x8*>(rs + tid * 8);
-    const u16x8 r1 = *reinterpret_cast<const u16x8*>(rs + kBE + tid * 8);
-    const u16x8 r2 = *reinterpret_cast<const u16x8*>(rs + 2 * kBE + tid * 8);
-    const u16x8 r3 = *reinterpret_cast<const u16x8*>(rs + 3 * kBE + tid * 8);
+    const u16x4v r0 = *reinterpret_cast<const u16x4v*>(rs + boff);
+    const u16x4v r1 = *reinterpret_cast<const u16x4v*>(rs + kBE + boff);
+    const u16x4v r2 = *reinterpret_cast<const u16x4v*>(rs + 2 * kBE + boff);
+    const u16x4v r3 = *reinterpret_cast<const u16x4v*>(rs + 3 * kBE + boff);
     // xf = F, yu = fY U / 2 (unscaled), shared by the chains
-    float xf[8], yu[8];
+    float xf[4], yu[4];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
+    for (int e = 0; e < 4; ++e) {
       yu[e] = (to_f<H>(r0[e]) + to_f<H>(r1[e])) * hfu;
       xf[e] = (to_f<H>(r2[e]) + to_f<H>(r3[e])) * inv_sf;
     }
-    const float* cf = cs + 8 * bq;   // wave-uniform: broadcast reads
+    const float* cf = cs + 4 * bq;   // wave-uniform: broadcast reads
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
-      const float4 c1a = *reinterpret_cast<const float4*>(cf + (c * 4 + 0) * kBK);
-      const float4 c1b = *reinterpret_cast<const float4*>(cf + (c * 4 + 0) * kBK + 4);
-      const float4 c2a = *reinterpret_cast<const float4*>(cf + (c * 4 + 1) * kBK);
-      const float4 c2b = *reinterpret_cast<const float4*>(cf + (c * 4 + 1) * kBK + 4);
-      const float c1v[8] = {c1a.x, c1a.y, c1a.z, c1a.w, c1b.x, c1b.y, c1b.z, c1b.w};
-      const float c2v[8] = {c2a.x, c2a.y, c2a.z, c2a.w, c2b.x, c2b.y, c2b.z, c2b.w};
-      u16x8 bh, bl;
+      const float4 c1 = *reinterpret_cast<const float4*>(cf + (c * 4 + 0) * kBK);
+      const float4 c2 = *reinterpret_cast<const float4*>(cf + (c * 4 + 1) * kBK);
+      const float c1v[4] = {c1.x, c1.y, c1.z, c1.w};
+      const float c2v[4] = {c2.x, c2.y, c2.z, c2.w};
+      u16x4v bh, bl;
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
+      for (int e = 0; e < 4; ++e) {
         const float b = c1v[e] * xf[e] + c2v[e] * yu[e];
         const u16 hb = from_f<H>(b);
         bh[e] = hb;
         bl[e] = from_f<H>(b - to_f<H>(hb));
       }
-      *reinterpret_cast<u16x8*>(b_slot + (2 * c) * kBE + tid * 8) = bh;
-      *reinterpret_cast<u16x8*>(b_slot + (2 * c + 1) * kBE + tid * 8) = bl;
+      *reinterpret_cast<u16x4v*>(b_slot + (2 * c) * kBE + boff) = bh;
+      *reinterpret_cast<u16x4v*>(b_slot + (2 * c + 1) * kBE + boff) = bl;
       if constexpr (DIAG) {
         const float4 aa = *reinterpret_cast<const float4*>(cf + (c * 4 + 2) * kBK);
-        const float4 ab = *reinterpret_cast<const float4*>(cf + (c * 4 + 2) * kBK + 4);
-        const float4 ba = *reinterpret_cast<const float4*>(cf + (c * 4 + 3) * kBK);
-        const float4 bb = *reinterpret_cast<const float4*>(cf + (c * 4 + 3) * kBK + 4);
-        const float av[8] = {aa.x, aa.y, aa.z, aa.w, ab.x, ab.y, ab.z, ab.w};
-        const float bv[8] = {ba.x, ba.y, ba.z, ba.w, bb.x, bb.y, bb.z, bb.w};
+        const float4 bb = *reinterpret_cast<const float4*>(cf + (c * 4 + 3) * kBK);
+        const float av[4] = {aa.x, aa.y, aa.z, aa.w};
+        const float bv[4] = {bb.x, bb.y, bb.z, bb.w};
         float fsum = 0.f;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) fsum += av[e] * xf[e] + 2.0f * bv[e] * yu[e];
+        for (int e = 0; e < 4; ++e) fsum += av[e] * xf[e] + bv[e] * (2.0f * yu[e]);
         fl_acc[c] += live * (double)fsum;
       }
     }
   };
+  (void)fyu;
 
-  v16f acc[NC][4];
+  v16f acc[NC][2];
 #pragma unroll
-  for (int c = 0; c < NC; ++c)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[c][i] = v16f{};
+  for (int c = 0; c < NC; ++c) acc[c][0] = acc[c][1] = v16f{};
 
-  typedef vec AFrag[4][2];
+  typedef vec AFrag[2][2];
   auto read_a = [&](const u16* ca, AFrag& fa) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int o = frag_off(i, lane);
+    for (int i = 0; i < 2; ++i) {
+      const int o = frag_off(2 * wr + i, lane);
       fa[i][0] = *reinterpret_cast<const vec*>(ca + o);
       fa[i][1] = *reinterpret_cast<const vec*>(ca + kBE + o);
     }
   };
-  // one K step t from ring position K % 4 (B slot K % 2), A fragments already
-  // in `cur`; those of step t + 1 are read into `nxt`.  Each step issues
-  // exactly 7 loads per wave (step t + 3, clamped), so vmcnt(7) at its end
+  // one K step t from ring position K % 4 (B slot K % 2).  Each step issues
+  // exactly 4 loads per wave (step t + 3, clamped), so vmcnt(4) at its end
   // means everything issued before it has landed: A of t + 1 and the raw
   // blocks and coefficients of t + 2.
-  auto step = [&](auto is_diag, auto kk, long t, AFrag& cur, AFrag& nxt) {
+  auto step = [&](auto is_diag, auto kk, long t) {
     constexpr int K = decltype(kk)::value;
     lds_barrier();   // B of step t stored; ring position (K + 3) % 4 free
     issue(t + 3, (K + 3) & 3);
     const u16* cb = sB[K & 1];
     vec fb[NC][2];
-    const int ob = frag_off(wave, lane);
+    const int ob = frag_off(wc, lane);
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       fb[c][0] = *reinterpret_cast<const vec*>(cb + (2 * c) * kBE + ob);
       fb[c][1] = *reinterpret_cast<const vec*>(cb + (2 * c + 1) * kBE + ob);
     }
-    read_a(sA[(K + 1) & 3], nxt);
+    AFrag cur;
+    read_a(sA[K], cur);
 #pragma unroll
     for (int pr = 0; pr < 3; ++pr)
 #pragma unroll
       for (int c = 0; c < NC; ++c)
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 2; ++i)
           acc[c][i] = Mma<H>::mma(cur[i][pr == 2 ? 1 : 0], fb[c][pr == 1 ? 1 : 0], acc[c][i]);
     // B of step t + 1 (clamped at the end of the range) into the other B slot
     build(is_diag, sR[(K + 1) & 3], sC[(K + 1) & 3], sB[(K + 1) & 1], t + 1 < n ? 1.0 : 0.0);
-    TDS_WAIT_VMCNT(7);
+    TDS_WAIT_VMCNT(4);
   };
 
   auto run = [&](auto is_diag) {
-    // prologue: steps 0..2 requested; B of step 0 built, A fragments of
-    // step 0 read
+    // prologue: steps 0..2 requested; B of step 0 built
     issue(0, 0);
     issue(1, 1);
     issue(2, 2);
     TDS_WAIT_VMCNT(0);
     lds_barrier();
     build(is_diag, sr0, sc0, sb0, 1.0);
-    AFrag f0, f1;
-    read_a(sa0, f0);
     for (long t = 0; t < n; t += 4) {
-      step(is_diag, ic<0>{}, t, f0, f1);
-      step(is_diag, ic<1>{}, t + 1, f1, f0);
-      step(is_diag, ic<2>{}, t + 2, f0, f1);
-      step(is_diag, ic<3>{}, t + 3, f1, f0);
+      step(is_diag, ic<0>{}, t);
+      step(is_diag, ic<1>{}, t + 1);
+      step(is_diag, ic<2>{}, t + 2);
+      step(is_diag, ic<3>{}, t + 3);
     }
   };
   if (n > 0) {
@@ -812,23 +843,24 @@ __global__ __launch_bounds__(256, 1) void outer_split(
   for (int c = 0; c < NC; ++c) {
     float* zp = zpart_all + ((long)(chain0 + c) * ksplit + ks) * np * np;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int mm = 32 * i + 8 * (r >> 2) + 4 * lh + (r & 3);
-        const int nn = 32 * wave + lr;
+        const int mm = 64 * wr + 32 * i + 8 * (r >> 2) + 4 * lh + (r & 3);
+        const int nn = 32 * wc + lr;
         zp[(long)(mu0 + mm) * np + nu0 + nn] = acc[c][i][r];
       }
   }
   if (diag) {
-    double* red = reinterpret_cast<double*>(sb0);  // [NC][2][128]
+    double* red = reinterpret_cast<double*>(sb0);  // [NC][4 quarters][128]
 #pragma unroll
-    for (int c = 0; c < NC; ++c) red[(c * 2 + (tid >> 7)) * kBR + brow] = fl_acc[c];
+    for (int c = 0; c < NC; ++c) red[(c * 4 + bq) * kBR + brow] = fl_acc[c];
     lds_barrier();
-    for (int e = tid; e < NC * kBR; e += 256) {
+    for (int e = tid; e < NC * kBR; e += 512) {
       const int c = e / kBR, nn = e % kBR;
+      const double* q = red + c * 4 * kBR + nn;
       fpart_all[((long)(chain0 + c) * ksplit + ks) * np + nu0 + nn] =
-          red[(c * 2 + 0) * kBR + nn] + red[(c * 2 + 1) * kBR + nn];
+          (q[0] + q[kBR]) + (q[2 * kBR] + q[3 * kBR]);
     }
   }
 }
@@ -901,6 +933,7 @@ int ensure_split(Ctx* c, int prec) {
     TDS_CHECK_HIP(hipMalloc((void**)&c->epi, sizeof(float) * kMaxChains * 2 * np));
     TDS_CHECK_HIP(hipMalloc((void**)&c->ysc, sizeof(double) * kMaxChains));
     TDS_CHECK_HIP(hipMalloc((void**)&c->cmax, sizeof(double) * kMaxChains * 2));
+    TDS_CHECK_HIP(hipMalloc((void**)&c->pmax, sizeof(double) * kMaxChains * kPartBlocks * 2));
     TDS_CHECK_HIP(hipMalloc((void**)&c->coef32, sizeof(float) * kMaxChains * Mp * 4));
     TDS_CHECK_HIP(hipMalloc((void**)&c->wpart, sizeof(double) * kMaxChains * (np / kBR) * Mp * 4));
     const int ks = choose_ksplit_split(np / kBR, Mp);
@@ -951,7 +984,7 @@ template <typename H, int NC>
 static void launch_outer(Ctx* c, const SplitStore& st, int chain0, hipStream_t s) {
   const int ntile = c->np / kBR;
   hipLaunchKernelGGL((outer_split<H, NC>), dim3((unsigned)(ntile * ntile * c->ksplit_s)),
-                     dim3(256), 0, s, (const u16*)st.UhT, (const u16*)st.UlT, (const u16*)st.FhT,
+                     dim3(512), 0, s, (const u16*)st.UhT, (const u16*)st.UlT, (const u16*)st.FhT,
                      (const u16*)st.FlT, c->coef32, st.su, st.sf, c->fy, c->np, c->Mp,
                      c->ksplit_s, ntile, chain0, c->zpart32, c->fpart_s);
 }
@@ -962,9 +995,11 @@ static int launch_all(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* e
   const int np = c->np;
   const long Mp = c->Mp;
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(ymax_kernel, dim3(nc), dim3(256), 0, s, c->Y1, st.su, np, c->ysc);
+  hipLaunchKernelGGL(ymax_part_kernel, dim3(kYmaxBlocks, nc), dim3(256), 0, s, c->Y1, st.su, np,
+                     c->pmax);
+  hipLaunchKernelGGL(ymax_kernel, dim3(nc), dim3(64), 0, s, c->pmax, kYmaxBlocks, c->ysc);
   hipLaunchKernelGGL(prep_y<H>, dim3(np, nc), dim3(256), 0, s, c->Y1, c->R1, st.su, c->fy, np,
-                     c->ysc, (u16*)c->Y1h, (u16*)c->Y1l, c->epi, c->cmax);
+                     c->ysc, (u16*)c->Y1h, (u16*)c->Y1l, c->epi);
   // chains in groups of up to 3 per pass
   for (int c0 = 0; c0 < nc;) {
     const int g = nc - c0 >= 3 ? 3 : nc - c0;
@@ -976,11 +1011,12 @@ static int launch_all(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* e
     c0 += g;
   }
   const int nct = np / kBR;
-  hipLaunchKernelGGL(finalize_coef, dim3((unsigned)((Mp + 255) / 256), nc), dim3(256), 0, s,
-                     c->wpart, c->rho_img, c->ysc, Mp, nct, apply_d4, (double*)nullptr, c->coef32,
-                     c->cmax);
-  hipLaunchKernelGGL(bscale_kernel, dim3(nc), dim3(256), 0, s, c->cmax, st.umax, st.fmax, c->fy, np,
-                     c->sbsc);
+  const long fin_blocks_l = (Mp + 255) / 256;
+  const int fin_blocks = fin_blocks_l < kPartBlocks ? (int)fin_blocks_l : kPartBlocks;
+  hipLaunchKernelGGL(finalize_coef, dim3((unsigned)fin_blocks, nc), dim3(256), 0, s, c->wpart,
+                     c->rho_img, c->ysc, Mp, nct, apply_d4, (double*)nullptr, c->coef32, c->pmax);
+  hipLaunchKernelGGL(bscale_kernel, dim3(nc), dim3(256), 0, s, c->pmax, fin_blocks, st.umax,
+                     st.fmax, c->fy, np, c->cmax, c->sbsc);
   hipLaunchKernelGGL(scale_coef_kernel, dim3((unsigned)((Mp + 255) / 256), nc), dim3(256), 0, s,
                      c->sbsc, np, Mp / kBK, c->coef32);
   if (ev) (void)hipEventRecord(ev[1], s);
