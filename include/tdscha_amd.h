@@ -148,6 +148,54 @@ int tdscha_continued_fraction(int n_a, const double* a, int n_b, const double* b
                               double* gf_out);
 
 /*
+ * Learned continued-fraction terminator: a fully connected FP32 network
+ * n_in -> width (n_hidden layers, SiLU) -> n_out (linear), trained and run
+ * by the library's own kernels.  No counterpart in the reference, whose only
+ * terminator is the constant one of Modules/DynamicalLanczos.py:5089-5122;
+ * the network's outputs extend the coefficients handed to that terminator.
+ * Parameters are one FP32 array [W0 (d1 x d0), b0 (d1), W1 (d2 x d1), ...],
+ * W row-major [out][in].  init_params may be NULL (all zero).
+ */
+typedef struct tdscha_mlp tdscha_mlp;
+int tdscha_mlp_create(tdscha_mlp** out, int n_in, int width, int n_hidden, int n_out,
+                      int max_batch, const float* init_params);
+void tdscha_mlp_destroy(tdscha_mlp* net);
+int tdscha_mlp_n_params(const tdscha_mlp* net, long* n_params);
+int tdscha_mlp_get_params(tdscha_mlp* net, float* params_out);
+/* reset_optimizer = 1 also clears the Adam moments and step count. */
+int tdscha_mlp_set_params(tdscha_mlp* net, const float* params, int reset_optimizer);
+/* Inference: y [batch, n_out] from x [batch, n_in] (host arrays). */
+int tdscha_mlp_forward(tdscha_mlp* net, int batch, const float* x, float* y);
+/* Loss kind 0: sum_o w_o (y_o - t_o)^2 averaged over the batch (w defaults to 1). */
+int tdscha_mlp_set_mse_weights(tdscha_mlp* net, const float* w);
+/*
+ * Loss kind 1 (n_out = 10): outputs z map to tail parameters
+ * p = out_mean + out_std * z in units of b0 (a_inf, b_inf, da[4], db[4]); the
+ * chain's K coefficients are extended by a_j = b0 (a_inf + da_j),
+ * b_j = b0 (b_inf + db_j), j = 0..3, and the terminator (a_inf, b_inf) b0;
+ * the loss is loss_scale * mean over samples and the n_w frequencies w of
+ * (S(w) / max|S_ref| - ref)^2 with S = -Im G the Wigner continued fraction
+ * (smearing eta) of Modules/DynamicalLanczos.py:5032-5162.
+ * Per-sample data: chains [n, 2K + 2] = (a[K], b[K], b0, 1 / max|S_ref|),
+ * ref [n, n_w] = S_ref / max|S_ref|.
+ */
+int tdscha_mlp_set_cf_loss(tdscha_mlp* net, int K, int n_w, const double* w, double smearing,
+                           const double* out_mean, const double* out_std, double loss_scale);
+/* Loss and parameter gradient of one batch (kind 0: target; kind 1: chains, ref). */
+int tdscha_mlp_loss_grad(tdscha_mlp* net, int kind, int batch, const float* x,
+                         const float* target, const double* chains, const double* ref,
+                         float* loss, float* grad_out);
+/* One Adam step on one batch (bias-corrected Adam, eps outside the square root). */
+int tdscha_mlp_adam_step(tdscha_mlp* net, int kind, int batch, const float* x,
+                         const float* target, const double* chains, const double* ref, float lr,
+                         float beta1, float beta2, float eps, float* loss);
+/* Minibatch Adam over a device-resident training set, reshuffled every epoch. */
+int tdscha_mlp_train(tdscha_mlp* net, int kind, long n_samples, const float* x,
+                     const float* target, const double* chains, const double* ref, int batch,
+                     int epochs, float lr, float beta1, float beta2, float eps,
+                     unsigned long long seed, float* epoch_loss);
+
+/*
  * Time the perturbed-average pipeline with HIP events on the context stream:
  * reps passes (after one warm-up) for n_chains chains at the given precision.
  * ms_out[4] = average ms of {weights GEMM, outer-product GEMM, K-split

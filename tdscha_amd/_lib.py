@@ -52,6 +52,23 @@ def _signatures(L):
     L.tdscha_device_free.argtypes = [_vp]
     L.tdscha_memcpy.argtypes = [_vp, _vp, C.c_size_t, C.c_int]
     L.tdscha_device_count.argtypes = [_ip]
+    L.tdscha_mlp_create.argtypes = [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    _vp]
+    L.tdscha_mlp_destroy.argtypes = [_vp]
+    L.tdscha_mlp_destroy.restype = None
+    L.tdscha_mlp_n_params.argtypes = [_vp, C.POINTER(C.c_long)]
+    L.tdscha_mlp_get_params.argtypes = [_vp, _vp]
+    L.tdscha_mlp_set_params.argtypes = [_vp, _vp, C.c_int]
+    L.tdscha_mlp_forward.argtypes = [_vp, C.c_int, _vp, _vp]
+    L.tdscha_mlp_set_mse_weights.argtypes = [_vp, _vp]
+    L.tdscha_mlp_set_cf_loss.argtypes = [_vp, C.c_int, C.c_int, _vp, C.c_double, _vp, _vp,
+                                         C.c_double]
+    L.tdscha_mlp_loss_grad.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp,
+                                       C.POINTER(C.c_float), _vp]
+    L.tdscha_mlp_adam_step.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, C.c_float,
+                                       C.c_float, C.c_float, C.c_float, C.POINTER(C.c_float)]
+    L.tdscha_mlp_train.argtypes = [_vp, C.c_int, C.c_long, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
+                                   C.c_float, C.c_float, C.c_float, C.c_float, C.c_ulonglong, _vp]
     return L
 
 
