@@ -1,0 +1,116 @@
+"""CPU pins of the q-space oracle (oracle/qspace_oracle.py)."""
+import numpy as np
+import pytest
+
+from oracle import lanczos_oracle as LO
+from oracle import qspace_oracle as QO
+from tdscha_amd import symmetry as SY
+from tdscha_amd.qspace import BlochModes, bloch_ensemble, minus_q, pair_map
+from tdscha_amd.rocksalt import Ensemble, RockSalt
+
+
+def _system(L=2, N=40, T=300.0, seed=3, k3=-2.0, k4=6.0):
+    ens = Ensemble(RockSalt(L=L, k3=k3, k4=k4), T, N, seed=seed)
+    modes, Xq, Yq = bloch_ensemble(ens)
+    return ens, modes, Xq, Yq
+
+
+def _qsys(modes, Xq, Yq, ens, iq, **kw):
+    pm, pairs = pair_map(modes.L, iq)
+    return QO.QSystem(Xq, Yq, modes.w_q, modes.valid_modes_q, ens.rho, ens.T, pm, pairs, iq,
+                      **kw)
+
+
+def _translations(ens):
+    m = ens.model
+    ops = []
+    for t in m.translations:
+        irt = SY.atom_permutation(m.positions, m.lattice, np.eye(3), m.positions[0],
+                                  np.asarray(t, float))
+        ops.append(ens.pols.T @ SY.cartesian_operator(irt, np.eye(3)) @ ens.pols)
+    return np.array(ops)
+
+
+def test_pair_map_momentum_conservation():
+    L = 4
+    for iq in (0, 5, 21, 63):
+        pm, pairs = pair_map(L, iq)
+        g = np.array([(i, j, k) for i in range(L) for j in range(L) for k in range(L)])
+        assert np.all(np.mod(g + g[pm] - g[iq], L) == 0)
+        assert len(set(i for p in pairs for i in p)) == 64
+    assert np.array_equal(minus_q(L)[minus_q(L)], np.arange(64))
+
+
+@pytest.mark.parametrize("band", [3, 5])
+def test_gamma_chain_equals_real_space_with_translation_images(band):
+    """The q-space L at q_pert = Gamma is the real-space L averaged over the
+    supercell translations, restricted to the Gamma sector: Lanczos
+    coefficients from the same starting mode agree."""
+    ens, modes, Xq, Yq = _system()
+    qs = _qsys(modes, Xq, Yq, ens, 0)
+    psi_q = np.zeros(qs.psi_size, np.complex128)
+    psi_q[band] = 1.0
+    aq, bq, cq, Qq = QO.hermitian_lanczos(qs.apply_full_L, psi_q, qs.mask(), 14)
+
+    n = len(ens.w)
+    E = modes.basis()
+    orc = LO.OracleSystem(ens.X, ens.Y, ens.w, ens.rho, ens.T, _translations(ens))
+    psi_r = np.zeros(LO.len_psi(n))
+    col = E[:, band]                                   # Gamma column of band `band`
+    assert np.abs(col.imag).max() < 1e-14
+    psi_r[:n] = ens.pols.T @ col.real
+    assert np.linalg.norm(psi_r[:n]) == pytest.approx(1.0, abs=1e-12)
+    ar, br, cr, _ = QO.hermitian_lanczos(orc.apply_full_L, psi_r, LO.mask_dot_wigner(n), 14)
+    print("a q-space", aq[:4], "real", ar[:4])
+    np.testing.assert_allclose(aq, ar, rtol=1e-9)
+    np.testing.assert_allclose(bq, br, rtol=1e-9)
+    np.testing.assert_allclose(bq, cq, rtol=1e-12)
+    G = np.real(np.conj(Qq) @ (Qq * qs.mask()).T)
+    assert np.abs(G - np.eye(len(Qq))).max() < 1e-10
+
+
+def test_harmonic_chain_closed_form():
+    ens, modes, Xq, Yq = _system()
+    iq = 3
+    qs = _qsys(modes, Xq, Yq, ens, iq, ignore_v3=True, ignore_v4=True)
+    psi = np.zeros(qs.psi_size, np.complex128)
+    psi[2] = 1.0
+    a, b, c, _ = QO.hermitian_lanczos(qs.apply_full_L, psi, qs.mask(), 5)
+    assert len(a) == 1 and len(b) == 0
+    assert a[0] == pytest.approx(-modes.w_q[2, iq] ** 2, rel=1e-14)
+
+
+@pytest.mark.parametrize("L,iq,band,N,steps", [(2, 1, 0, 40, 10), (2, 5, 4, 40, 10),
+                                                 (2, 7, 2, 40, 10)])
+def test_chain_at_finite_q_equals_real_space_complex_chain(L, iq, band, N, steps):
+    """At q_pert != Gamma the same equivalence holds for the complex Bloch mode
+    as starting vector: the real-space operator acts on real and imaginary
+    parts (L(Re v) + i L(Im v)) and the Lanczos uses real parts of the masked
+    products.  On the 2x2x2 grid (the reference's own q-space test system)
+    every q equals -q; on larger grids the reference's pairing
+    q1 + q2 = q_pert with conj(x_q1) x_q2 products is not the momentum-
+    conserving one, and the oracle follows the reference there."""
+    ens, modes, Xq, Yq = _system(L=L, N=N)
+    qs = _qsys(modes, Xq, Yq, ens, iq)
+    psi_q = np.zeros(qs.psi_size, np.complex128)
+    psi_q[band] = 1.0
+    aq, bq, _, _ = QO.hermitian_lanczos(qs.apply_full_L, psi_q, qs.mask(), steps)
+    n = len(ens.w)
+    orc = LO.OracleSystem(ens.X, ens.Y, ens.w, ens.rho, ens.T, _translations(ens))
+    mv = lambda v: orc.apply_full_L(v.real) + 1j * orc.apply_full_L(v.imag)
+    psi_r = np.zeros(LO.len_psi(n), np.complex128)
+    psi_r[:n] = ens.pols.T @ modes.basis()[:, iq * modes.n_bands + band]
+    ar, br, _, _ = QO.hermitian_lanczos(mv, psi_r, LO.mask_dot_wigner(n), steps)
+    np.testing.assert_allclose(aq, ar, rtol=1e-10)
+    np.testing.assert_allclose(bq, br, rtol=1e-10)
+
+
+def test_mgs_and_cgs2_reorthogonalisation_agree():
+    ens, modes, Xq, Yq = _system(N=30)
+    qs = _qsys(modes, Xq, Yq, ens, 0)
+    psi = np.zeros(qs.psi_size, np.complex128)
+    psi[4] = 1.0
+    a1, b1, _, _ = QO.hermitian_lanczos(qs.apply_full_L, psi, qs.mask(), 30, mgs=True)
+    a2, b2, _, _ = QO.hermitian_lanczos(qs.apply_full_L, psi, qs.mask(), 30, mgs=False)
+    np.testing.assert_allclose(a1, a2, rtol=1e-10)
+    np.testing.assert_allclose(b1, b2, rtol=1e-10)
