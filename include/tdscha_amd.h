@@ -148,6 +148,65 @@ int tdscha_continued_fraction(int n_a, const double* a, int n_b, const double* b
                               double* gf_out);
 
 /*
+ * q-space (Bloch basis) context: replaces the ensemble arguments of
+ * get_perturb_averages_qspace (Modules/tdscha_qspace.jl:639-712) and the
+ * q-space attributes of QSpaceLanczos (Modules/QSpaceLanczos.py:150-295).
+ *   X_q, Y_q  complex128 [n_q, n_configs, n_bands] (interleaved re, im),
+ *             reference layout X_q[iq, config, nu]
+ *   w_q       [n_bands, n_q] frequencies (Ry); valid_modes_q [n_bands, n_q]
+ *             (0 = masked acoustic band); rho [n_configs]; T (K)
+ * n_bands must be 3 or 6.  Point-group images are not supported (one
+ * symmetry, the identity, as QSpaceLanczos.prepare_symmetrization(no_sym=True)).
+ */
+typedef struct tdscha_qctx tdscha_qctx;
+int tdscha_qctx_create(tdscha_qctx** out, int n_q, int n_bands, int n_configs, const double* X_q,
+                       const double* Y_q, const double* w_q, const int* valid_modes_q,
+                       const double* rho, double T);
+void tdscha_qctx_destroy(tdscha_qctx* ctx);
+
+/* psi length of the layout built from one q_pair_map (QL:313-343). */
+int tdscha_qspace_psi_size(int n_q, int n_bands, const int* q_pair_map, int* psi_size);
+
+/*
+ * QSpaceLanczos.apply_full_L (Modules/QSpaceLanczos.py:870-893) for n_chains
+ * (1..256) vectors, chain c perturbed at iq_pert[c] with pair map
+ * q_pair_map[c, n_q] (q1 + q2 = q_pert + G, QL:297-331).  psi and out hold
+ * the chains' complex vectors one after the other (interleaved re, im), each
+ * in the reference layout of its own pair map.
+ */
+int tdscha_qspace_apply_L(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                          const int* q_pair_map, const double* psi, double* out,
+                          int ignore_harmonic, int ignore_v3, int ignore_v4);
+
+/*
+ * Hermitian Lanczos with full reorthogonalisation (QSpaceLanczos.run_FT with
+ * reorthogonalize=True, QL:967-1182) for n_chains chains at once, device
+ * resident.  Chain c stops when |b| < tol or after caps[c] steps (caps may
+ * be NULL: n_steps).  use_graph = 1 captures the whole run as one hipGraph and
+ * launches it once.  a_out, b_out, c_out (may be NULL) [n_chains, n_steps];
+ * n_done[c] = number of a coefficients (b has n_done[c] entries, or one fewer
+ * when the chain stopped on |b| < tol).
+ */
+int tdscha_qspace_lanczos_run(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                              const int* q_pair_map, const double* psi0, int n_steps,
+                              const int* caps, double tol, int ignore_harmonic, int ignore_v3,
+                              int ignore_v4, int use_graph, double* a_out, double* b_out,
+                              double* c_out, int* n_done, double* elapsed_ms);
+
+/*
+ * Tile choices of the q-space kernels: configurations per workgroup of the
+ * weights kernel (64, 128, 256), configurations per workgroup of the d2v
+ * kernel (0 = automatic) and basis vectors per workgroup of the
+ * reorthogonalisation (8, 16, 32).  tdscha_qspace_autotune times one L
+ * application of the given batch for every weights/d2v candidate (reps
+ * repetitions each), keeps the fastest and returns it in tiles_out[3].
+ */
+int tdscha_qspace_set_tiles(tdscha_qctx* ctx, int weights_tile, int d2v_chunk, int reorth_jtile);
+int tdscha_qspace_get_tiles(const tdscha_qctx* ctx, int* tiles_out);
+int tdscha_qspace_autotune(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                           const int* q_pair_map, int reps, int* tiles_out, double* best_ms);
+
+/*
  * Learned continued-fraction terminator: a fully connected FP32 network
  * n_in -> width (n_hidden layers, SiLU) -> n_out (linear), trained and run
  * by the library's own kernels.  No counterpart in the reference, whose only

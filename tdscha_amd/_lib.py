@@ -52,7 +52,21 @@ def _signatures(L):
     L.tdscha_device_free.argtypes = [_vp]
     L.tdscha_memcpy.argtypes = [_vp, _vp, C.c_size_t, C.c_int]
     L.tdscha_device_count.argtypes = [_ip]
-    L.tdscha_mlp_create.argtypes = [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+    L.tdscha_qctx_create.argtypes = [C.POINTER(_vp), C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
+                                     _vp, _vp, C.c_double]
+    L.tdscha_qctx_destroy.argtypes = [_vp]
+    L.tdscha_qctx_destroy.restype = None
+    L.tdscha_qspace_psi_size.argtypes = [C.c_int, C.c_int, _vp, _ip]
+    L.tdscha_qspace_apply_L.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
+                                        C.c_int]
+    L.tdscha_qspace_lanczos_run.argtypes = [_vp, C.c_int, _vp, _vp, _vp, C.c_int, _vp, C.c_double,
+                                            C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
+                                            _vp, C.POINTER(C.c_double)]
+    L.tdscha_qspace_set_tiles.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.tdscha_qspace_get_tiles.argtypes = [_vp, _vp]
+    L.tdscha_qspace_autotune.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, _vp,
+                                         C.POINTER(C.c_double)]
+    L.tdscha_mlp_create.argtypes =[C.POINTER(_vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     _vp]
     L.tdscha_mlp_destroy.argtypes = [_vp]
     L.tdscha_mlp_destroy.restype = None
