@@ -135,11 +135,13 @@ __global__ void finalize_pa(const double* __restrict__ zsum_all, const double* _
       norm * (fy[mu] * z[(long)mu * np + nu] + fy[nu] * z[(long)nu * np + mu]);
 }
 
-// Batched FP64 GEMM C = op(A) op(B) on MFMA (v_mfma_f64_16x16x4f64): 64 x 64
-// tiles, 4 waves of 32 x 32 (2 x 2 blocks of 16 x 16), K in chunks of 16
-// through LDS.  ta / tb select A^T / B^T; the LDS fill is coalesced along the
-// contiguous index of each operand.  Operand layout of the 16x16x4 FP64 MFMA:
-// A[i][k]: i = lane % 16, k = lane / 16; B likewise with j; result
+// C = op(A) op(B), FP64, batched (blockIdx.z), on v_mfma_f64_16x16x4f64.
+// A 32 x 32 output tile per work-group; the four waves split each 64-deep K
+// chunk (16 rows each) and their partial tiles are summed in LDS in wave
+// order at the end (fixed order).  The next K chunk is loaded into registers
+// while the current one is multiplied.  ta / tb select A^T / B^T; global
+// loads run along the contiguous index of each operand.  Operand layout of
+// the MFMA: A[i][k]: i = lane % 16, k = lane / 16; B likewise with j;
 // D[i][j]: i = 4 r + lane / 16, j = lane % 16 (r = 0..3).
 __global__ __launch_bounds__(256) void gemm_f64(int M, int N, int K, const double* __restrict__ A,
                                                 long lda, long sA, int ta,
@@ -149,39 +151,56 @@ __global__ __launch_bounds__(256) void gemm_f64(int M, int N, int K, const doubl
   A += b * sA;
   B += b * sB;
   C += b * sC;
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int m0 = blockIdx.y * 32, n0 = blockIdx.x * 32;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
-  __shared__ double As[16][65];
-  __shared__ double Bs[16][65];
+  __shared__ double As[64][33];
+  __shared__ double Bs[64][33];
+  __shared__ double red[3][32 * 32];
+  double ra[8], rb[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = tid + 256 * r;
+      int mm, kk;
+      if (ta) { kk = e >> 5; mm = e & 31; } else { mm = e >> 6; kk = e & 63; }
+      const int gm = m0 + mm, gk = k0 + kk;
+      ra[r] = (gm < M && gk < K) ? (ta ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk]) : 0.0;
+      int nn;
+      if (tb) { nn = e >> 6; kk = e & 63; } else { kk = e >> 5; nn = e & 31; }
+      const int gn = n0 + nn, gk2 = k0 + kk;
+      rb[r] = (gn < N && gk2 < K) ? (tb ? B[(long)gn * ldb + gk2] : B[(long)gk2 * ldb + gn]) : 0.0;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int e = tid + 256 * r;
+      int mm, kk;
+      if (ta) { kk = e >> 5; mm = e & 31; } else { mm = e >> 6; kk = e & 63; }
+      As[kk][mm] = ra[r];
+      int nn;
+      if (tb) { nn = e >> 6; kk = e & 63; } else { kk = e >> 5; nn = e & 31; }
+      Bs[kk][nn] = rb[r];
+    }
+  };
   v4d acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) acc[i][j] = v4d{0, 0, 0, 0};
-  for (int k0 = 0; k0 < K; k0 += 16) {
-    for (int e = tid; e < 16 * 64; e += 256) {
-      // A tile element (m0 + mm, k0 + kk)
-      int mm, kk;
-      if (ta) { kk = e >> 6; mm = e & 63; } else { mm = e >> 4; kk = e & 15; }
-      int gm = m0 + mm, gk = k0 + kk;
-      As[kk][mm] = (gm < M && gk < K) ? (ta ? A[(long)gk * lda + gm] : A[(long)gm * lda + gk]) : 0.0;
-      // B tile element (k0 + kk, n0 + nn)
-      int nn;
-      if (tb) { nn = e >> 4; kk = e & 15; } else { kk = e >> 6; nn = e & 63; }
-      const int gn = n0 + nn;
-      gk = k0 + kk;
-      Bs[kk][nn] = (gn < N && gk < K) ? (tb ? B[(long)gn * ldb + gk] : B[(long)gk * ldb + gn]) : 0.0;
-    }
+  load(0);
+  for (int k0 = 0; k0 < K; k0 += 64) {
+    store();
     __syncthreads();
+    if (k0 + 64 < K) load(k0 + 64);
 #pragma unroll
     for (int ks = 0; ks < 4; ++ks) {
-      const int k = 4 * ks + (lane >> 4);
+      const int k = 16 * wave + 4 * ks + (lane >> 4);
       double a[2], bb[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[k][32 * wr + 16 * i + (lane & 15)];
+      for (int i = 0; i < 2; ++i) a[i] = As[k][16 * i + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bb[j] = Bs[k][32 * wc + 16 * j + (lane & 15)];
+      for (int j = 0; j < 2; ++j) bb[j] = Bs[k][16 * j + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -190,22 +209,36 @@ __global__ __launch_bounds__(256) void gemm_f64(int M, int N, int K, const doubl
     }
     __syncthreads();
   }
+  if (wave > 0) {
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < 2; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gm = m0 + 32 * wr + 16 * i + 4 * r + (lane >> 4);
-        const int gn = n0 + 32 * wc + 16 * j + (lane & 15);
-        if (gm < M && gn < N) C[(long)gm * ldc + gn] = acc[i][j][r];
-      }
+        for (int r = 0; r < 4; ++r)
+          red[wave - 1][(16 * i + 4 * r + (lane >> 4)) * 32 + 16 * j + (lane & 15)] = acc[i][j][r];
+  }
+  __syncthreads();
+  if (wave == 0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int li = 16 * i + 4 * r + (lane >> 4), lj = 16 * j + (lane & 15);
+          const double v = acc[i][j][r] + red[0][li * 32 + lj] + red[1][li * 32 + lj] +
+                           red[2][li * 32 + lj];
+          const int gm = m0 + li, gn = n0 + lj;
+          if (gm < M && gn < N) C[(long)gm * ldc + gn] = v;
+        }
+  }
 }
 
 static void gemm(int batch, int M, int N, int K, const double* A, long lda, long sA, int ta,
                  const double* B, long ldb, long sB, int tb, double* C, long ldc, long sC,
                  hipStream_t s) {
-  dim3 g((N + 63) / 64, (M + 63) / 64, batch);
+  dim3 g((N + 31) / 32, (M + 31) / 32, batch);
   hipLaunchKernelGGL(gemm_f64, g, dim3(256), 0, s, M, N, K, A, lda, sA, ta, B, ldb, sB, tb, C, ldc,
                      sC);
 }

@@ -42,11 +42,15 @@ struct LanczosState {
   int* active = nullptr;     // [nc]
   int* cap = nullptr;        // [nc]
   int* ndone = nullptr;      // [nc] number of a coefficients
+  double* upart = nullptr;   // [nc, jsplit, len] basis-range partial updates (short vectors)
+  int nslice = 1;            // element slices of the projections
+  int jsplit = 1;            // basis-vector ranges of the update
+  int jchunk = 1;
 };
 
 static void free_state(LanczosState& s) {
   void* p[] = {s.Q, s.qcur, s.Lq, s.r, s.mask, s.part, s.coef, s.a, s.b, s.scal, s.k, s.active,
-               s.cap, s.ndone};
+               s.cap, s.ndone, s.upart};
   for (void* x : p)
     if (x) (void)hipFree(x);
 }
@@ -123,56 +127,96 @@ __global__ void lanczos_residual(const double* __restrict__ part, long ps,
   r[c * len + i] = v;
 }
 
-// CGS: coefficients of r against Q_0..Q_k (partials), one block row per slice
-__global__ void reorth_partial(const double* __restrict__ Q, const double* __restrict__ r,
-                               const double* __restrict__ mask, long len, int kmax,
-                               const int* __restrict__ kk, const int* __restrict__ active,
-                               double* __restrict__ part) {
-  const int c = blockIdx.y;
+__device__ __forceinline__ double wave_sum64(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// CGS projections <Q_j, r>_M for j <= k: one work-group per (element slice,
+// tile of JT basis vectors, chain); partial sums part[c][j][slice]
+template <int JT>
+__global__ void __launch_bounds__(256) reorth_dots(const double* __restrict__ Q,
+                                                   const double* __restrict__ r,
+                                                   const double* __restrict__ mask, long len,
+                                                   int kmax, const int* __restrict__ kk,
+                                                   const int* __restrict__ active, int nslice,
+                                                   double* __restrict__ part) {
+  const int c = blockIdx.z, jt = blockIdx.y, sl = blockIdx.x, t = threadIdx.x;
   if (!active[c]) return;
   const int k = kk[c];
-  const long slice = (len + kRedBlocks - 1) / kRedBlocks;
-  const long lo = (long)blockIdx.x * slice;
-  long hi = lo + slice;
-  if (hi > len) hi = len;
-  __shared__ double red[kRedThreads];
-  const double* rc = r + c * len;
-  for (int j = 0; j <= k; ++j) {
-    const double* qj = Q + ((long)c * (kmax + 1) + j) * len;
-    double s = 0.0;
-    for (long i = lo + threadIdx.x; i < hi; i += blockDim.x) s += qj[i] * rc[i] * mask[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = kRedThreads / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0)
-      part[((long)c * (kmax + 1) + j) * kRedBlocks + blockIdx.x] = red[0];
-    __syncthreads();
+  const int j0 = jt * JT;
+  if (j0 > k) return;
+  const int nj = min(JT, k + 1 - j0);
+  const long slice = ((len + nslice - 1) / nslice + 255) / 256 * 256;
+  const long lo = (long)sl * slice;
+  const long hi = min(len, lo + slice);
+  double s[JT];
+#pragma unroll
+  for (int j = 0; j < JT; ++j) s[j] = 0.0;
+  const double* rc = r + (long)c * len;
+  const double* Qc = Q + ((long)c * (kmax + 1) + j0) * len;
+  for (long e = lo + t; e < hi; e += 256) {
+    const double rm = rc[e] * mask[e];
+#pragma unroll
+    for (int j = 0; j < JT; ++j)
+      if (j < nj) s[j] += Qc[(long)j * len + e] * rm;
   }
+  __shared__ double red[4][JT];
+#pragma unroll
+  for (int j = 0; j < JT; ++j) {
+    const double v = wave_sum64(s[j]);
+    if ((t & 63) == 0) red[t >> 6][j] = v;
+  }
+  __syncthreads();
+  if (t < nj)
+    part[((long)c * (kmax + 1) + j0 + t) * kRedBlocks + sl] =
+        red[0][t] + red[1][t] + red[2][t] + red[3][t];
 }
 
 __global__ void reorth_coef(const double* __restrict__ part, int kmax, const int* __restrict__ kk,
-                            const int* __restrict__ active, double* __restrict__ coef) {
+                            const int* __restrict__ active, int nslice,
+                            double* __restrict__ coef) {
   const int c = blockIdx.y;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (!active[c] || j > kk[c]) return;
-  coef[c * (kmax + 1) + j] = sum_partials(part + ((long)c * (kmax + 1) + j) * kRedBlocks);
+  const double* p = part + ((long)c * (kmax + 1) + j) * kRedBlocks;
+  double s = 0.0;
+  for (int i = 0; i < nslice; ++i) s += p[i];
+  coef[c * (kmax + 1) + j] = s;
 }
 
+// r -= sum_j coef_j Q_j.  jsplit == 1: in place.  jsplit > 1 (short vectors):
+// each grid slice z sums its range of basis vectors into upart[c][z], and
+// reorth_finish subtracts the ranges in order.
 __global__ void reorth_update(const double* __restrict__ Q, const double* __restrict__ coef,
                               long len, int kmax, const int* __restrict__ kk,
+                              const int* __restrict__ active, int jsplit, int jchunk,
+                              double* __restrict__ upart, double* __restrict__ r) {
+  const int c = blockIdx.y, z = blockIdx.z;
+  if (!active[c]) return;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const int k = kk[c];
+  const int ja = z * jchunk, jb = min(k + 1, ja + jchunk);
+  double v = 0.0;
+  for (int j = ja; j < jb; ++j)
+    v += coef[c * (kmax + 1) + j] * Q[((long)c * (kmax + 1) + j) * len + i];
+  if (jsplit == 1)
+    r[c * len + i] -= v;
+  else
+    upart[((long)c * jsplit + z) * len + i] = v;
+}
+
+__global__ void reorth_finish(const double* __restrict__ upart, long len, int jsplit,
                               const int* __restrict__ active, double* __restrict__ r) {
   const int c = blockIdx.y;
   if (!active[c]) return;
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;
-  const int k = kk[c];
-  double v = r[c * len + i];
-  for (int j = 0; j <= k; ++j)
-    v -= coef[c * (kmax + 1) + j] * Q[((long)c * (kmax + 1) + j) * len + i];
-  r[c * len + i] = v;
+  double v = 0.0;
+  for (int z = 0; z < jsplit; ++z) v += upart[((long)c * jsplit + z) * len + i];
+  r[c * len + i] -= v;
 }
 
 // b_k = ||r||_M ; stop or normalise into Q_{k+1} / qcur ; advance k
@@ -240,13 +284,18 @@ static void lanczos_step(Ctx* c, LanczosState& st, int reorth, int precision, in
   hipLaunchKernelGGL(lanczos_residual, gv, dim3(256), 0, s, st.part, ps, st.Q, st.qcur, st.Lq,
                      len, kmax, st.k, st.active, st.a, st.b, st.scal, st.r);
   if (reorth) {
+    constexpr int JT = 16;
     for (int pass = 0; pass < 2; ++pass) {
-      hipLaunchKernelGGL(reorth_partial, gr, dim3(kRedThreads), 0, s, st.Q, st.r, st.mask, len,
-                         kmax, st.k, st.active, st.part);
+      hipLaunchKernelGGL(reorth_dots<JT>, dim3(st.nslice, (kmax + JT) / JT, nc), dim3(256), 0, s,
+                         st.Q, st.r, st.mask, len, kmax, st.k, st.active, st.nslice, st.part);
       hipLaunchKernelGGL(reorth_coef, dim3((kmax + 1 + 63) / 64, nc), dim3(64), 0, s, st.part,
-                         kmax, st.k, st.active, st.coef);
-      hipLaunchKernelGGL(reorth_update, gv, dim3(256), 0, s, st.Q, st.coef, len, kmax, st.k,
-                         st.active, st.r);
+                         kmax, st.k, st.active, st.nslice, st.coef);
+      hipLaunchKernelGGL(reorth_update, dim3(gv.x, nc, st.jsplit), dim3(256), 0, s, st.Q,
+                         st.coef, len, kmax, st.k, st.active, st.jsplit, st.jchunk, st.upart,
+                         st.r);
+      if (st.jsplit > 1)
+        hipLaunchKernelGGL(reorth_finish, gv, dim3(256), 0, s, st.upart, len, st.jsplit,
+                           st.active, st.r);
     }
   }
   hipLaunchKernelGGL(dot_partial, gr, dim3(kRedThreads), 0, s, st.r, len, st.r, len, st.mask, len,
@@ -307,6 +356,14 @@ int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_
             hipMalloc((void**)&st.active, sizeof(int) * n_chains) == hipSuccess &&
             hipMalloc((void**)&st.cap, sizeof(int) * n_chains) == hipSuccess &&
             hipMalloc((void**)&st.ndone, sizeof(int) * n_chains) == hipSuccess;
+  // projections: element slices of >= 4096 entries (at most 64); updates of
+  // short vectors split over ranges of basis vectors so that enough
+  // work-groups run
+  st.nslice = (int)std::max(1L, std::min(64L, len / 4096));
+  st.jsplit = len < 32768 ? std::max(1, std::min(16, (n_steps + 1) / 32)) : 1;
+  st.jchunk = (n_steps + 1 + st.jsplit - 1) / st.jsplit;
+  if (ok && st.jsplit > 1)
+    ok = hipMalloc((void**)&st.upart, sizeof(double) * n_chains * st.jsplit * len) == hipSuccess;
   if (!ok) {
     free_state(st);
     set_error("tdscha_lanczos_run: device allocation failed");
