@@ -429,27 +429,33 @@ __global__ __launch_bounds__(256, 1) void weights_split(
     }
     lds_barrier();
     // two halves of 64 rows: every wave stores its column-quarter partials,
-    // then the four quarters are summed in a fixed order
+    // then the four quarters are summed in a fixed order.  The image loads of
+    // row group i + 1 (16 loads of 8 bytes per lane) are issued before row
+    // group i is multiplied, so their latency overlaps the arithmetic.
+    u16x4 ldb[2][4][4];
+    auto load_rows = [&](int i, u16x4 (&ld)[4][4]) {
+      const int row = 32 * i + lr;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int col = 32 * wave + 8 * q + 4 * lh;
+        const long a = ((long)rt * nkc + (c0 + col) / kBK) * kBE + blk_off(row, col % kBK);
+        ld[q][0] = *reinterpret_cast<const u16x4*>(UAh + a);
+        ld[q][1] = *reinterpret_cast<const u16x4*>(UAl + a);
+        ld[q][2] = *reinterpret_cast<const u16x4*>(FAh + a);
+        ld[q][3] = *reinterpret_cast<const u16x4*>(FAl + a);
+      }
+    };
+    load_rows(0, ldb[0]);
 #pragma unroll
     for (int half = 0; half < 2; ++half) {
 #pragma unroll
       for (int ii = 0; ii < 2; ++ii) {
         const int i = 2 * half + ii;
-        const int row = 32 * i + lr;
         double p[NC][4];
 #pragma unroll
         for (int c = 0; c < NC; ++c) p[c][0] = p[c][1] = p[c][2] = p[c][3] = 0.0;
-        // the image loads of this row first: 16 loads of 8 bytes
-        u16x4 ld[4][4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int col = 32 * wave + 8 * q + 4 * lh;
-          const long a = ((long)rt * nkc + (c0 + col) / kBK) * kBE + blk_off(row, col % kBK);
-          ld[q][0] = *reinterpret_cast<const u16x4*>(UAh + a);
-          ld[q][1] = *reinterpret_cast<const u16x4*>(UAl + a);
-          ld[q][2] = *reinterpret_cast<const u16x4*>(FAh + a);
-          ld[q][3] = *reinterpret_cast<const u16x4*>(FAl + a);
-        }
+        if (i < 3) load_rows(i + 1, ldb[(i + 1) & 1]);
+        u16x4 (&ld)[4][4] = ldb[i & 1];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int col = 32 * wave + 8 * q + 4 * lh;   // tile column of t = 0
