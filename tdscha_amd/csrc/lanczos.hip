@@ -40,6 +40,7 @@ struct LanczosState {
   double* scal = nullptr;    // [nc, 4]: a_k, b_k, scratch
   int* k = nullptr;          // [nc] current step
   int* active = nullptr;     // [nc]
+  int* act2 = nullptr;       // [nc] chains whose second Gram-Schmidt pass runs
   int* cap = nullptr;        // [nc]
   int* ndone = nullptr;      // [nc] number of a coefficients
   double* upart = nullptr;   // [nc, jsplit, len] basis-range partial updates (short vectors)
@@ -50,7 +51,7 @@ struct LanczosState {
 
 static void free_state(LanczosState& s) {
   void* p[] = {s.Q, s.qcur, s.Lq, s.r, s.mask, s.part, s.coef, s.a, s.b, s.scal, s.k, s.active,
-               s.cap, s.ndone, s.upart};
+               s.cap, s.ndone, s.upart, s.act2};
   for (void* x : p)
     if (x) (void)hipFree(x);
 }
@@ -219,6 +220,27 @@ __global__ void reorth_finish(const double* __restrict__ upart, long len, int js
   r[c * len + i] -= v;
 }
 
+// Second Gram-Schmidt pass only where the first one removed more than half
+// of the residual's squared norm (the Daniel-Gragg-Kaufman-Stewart test,
+// |r'| < |r| / sqrt(2)): the norm before the pass is kept in scal[c][2], the
+// norm after it is summed from partial row `slot`.
+__global__ void keep_norm(const double* __restrict__ part, long ps, int slot,
+                          const int* __restrict__ active, double* __restrict__ scal, int nc) {
+  const int c = threadIdx.x;
+  if (c >= nc || !active[c]) return;
+  scal[c * 4 + 2] = sum_partials(part + c * ps + (long)slot * kRedBlocks);
+}
+
+__global__ void second_pass_flags(const double* __restrict__ part, long ps, int slot,
+                                  const int* __restrict__ active,
+                                  const double* __restrict__ scal, int* __restrict__ act2,
+                                  int nc) {
+  const int c = threadIdx.x;
+  if (c >= nc) return;
+  const double after = sum_partials(part + c * ps + (long)slot * kRedBlocks);
+  act2[c] = (active[c] && after < 0.5 * scal[c * 4 + 2]) ? 1 : 0;
+}
+
 // b_k = ||r||_M ; stop or normalise into Q_{k+1} / qcur ; advance k
 __global__ void lanczos_advance(const double* __restrict__ part, long ps, long len, int kmax,
                                 double tol, int* __restrict__ kk, int* __restrict__ active,
@@ -284,18 +306,31 @@ static void lanczos_step(Ctx* c, LanczosState& st, int reorth, int precision, in
   hipLaunchKernelGGL(lanczos_residual, gv, dim3(256), 0, s, st.part, ps, st.Q, st.qcur, st.Lq,
                      len, kmax, st.k, st.active, st.a, st.b, st.scal, st.r);
   if (reorth) {
+    // classical Gram-Schmidt against the whole basis; the second pass runs
+    // for the chains whose first pass failed the norm test (flags in act2).
+    // Partial row kmax is never a basis row, so it holds the two norms.
     constexpr int JT = 16;
+    hipLaunchKernelGGL(dot_partial, gr, dim3(kRedThreads), 0, s, st.r, len, st.r, len, st.mask,
+                       len, st.part, ps, kmax);
+    hipLaunchKernelGGL(keep_norm, dim3(1), dim3(64), 0, s, st.part, ps, kmax, st.active, st.scal,
+                       nc);
     for (int pass = 0; pass < 2; ++pass) {
+      const int* act = pass == 0 ? st.active : st.act2;
+      if (pass == 1) {
+        hipLaunchKernelGGL(dot_partial, gr, dim3(kRedThreads), 0, s, st.r, len, st.r, len,
+                           st.mask, len, st.part, ps, kmax);
+        hipLaunchKernelGGL(second_pass_flags, dim3(1), dim3(64), 0, s, st.part, ps, kmax,
+                           st.active, st.scal, st.act2, nc);
+      }
       hipLaunchKernelGGL(reorth_dots<JT>, dim3(st.nslice, (kmax + JT) / JT, nc), dim3(256), 0, s,
-                         st.Q, st.r, st.mask, len, kmax, st.k, st.active, st.nslice, st.part);
+                         st.Q, st.r, st.mask, len, kmax, st.k, act, st.nslice, st.part);
       hipLaunchKernelGGL(reorth_coef, dim3((kmax + 1 + 63) / 64, nc), dim3(64), 0, s, st.part,
-                         kmax, st.k, st.active, st.nslice, st.coef);
+                         kmax, st.k, act, st.nslice, st.coef);
       hipLaunchKernelGGL(reorth_update, dim3(gv.x, nc, st.jsplit), dim3(256), 0, s, st.Q,
-                         st.coef, len, kmax, st.k, st.active, st.jsplit, st.jchunk, st.upart,
-                         st.r);
+                         st.coef, len, kmax, st.k, act, st.jsplit, st.jchunk, st.upart, st.r);
       if (st.jsplit > 1)
-        hipLaunchKernelGGL(reorth_finish, gv, dim3(256), 0, s, st.upart, len, st.jsplit,
-                           st.active, st.r);
+        hipLaunchKernelGGL(reorth_finish, gv, dim3(256), 0, s, st.upart, len, st.jsplit, act,
+                           st.r);
     }
   }
   hipLaunchKernelGGL(dot_partial, gr, dim3(kRedThreads), 0, s, st.r, len, st.r, len, st.mask, len,
@@ -354,6 +389,7 @@ int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_
             hipMalloc((void**)&st.scal, sizeof(double) * n_chains * 4) == hipSuccess &&
             hipMalloc((void**)&st.k, sizeof(int) * n_chains) == hipSuccess &&
             hipMalloc((void**)&st.active, sizeof(int) * n_chains) == hipSuccess &&
+            hipMalloc((void**)&st.act2, sizeof(int) * n_chains) == hipSuccess &&
             hipMalloc((void**)&st.cap, sizeof(int) * n_chains) == hipSuccess &&
             hipMalloc((void**)&st.ndone, sizeof(int) * n_chains) == hipSuccess;
   // projections: element slices of >= 4096 entries (at most 64); updates of
