@@ -696,11 +696,14 @@ __global__ __launch_bounds__(512, 1) void outer_split(
   const long g0 = 4 * ((long)ks * nunit / ksplit);
   const long n = 4 * ((long)(ks + 1) * nunit / ksplit) - g0;
 
-  // builder role: thread t owns images 4 bq .. 4 bq + 3 (bq = t / 128, one
-  // value per wave) of row nu0 + t % 128 of every B block: 8 bytes at
-  // element (bq / 2) * 1024 + row * 8 + 4 (bq % 2) of the chunk-major block
-  const int brow = tid & (kBR - 1);
-  const int bq = tid >> 7;
+  // builder role: thread t owns images 4 bq .. 4 bq + 3 of row nu0 + brow of
+  // every B block: 8 bytes at element (bq / 2) * 1024 + row * 8 + 4 (bq % 2)
+  // of the chunk-major block.  The two halves of a row's 8-image chunk sit on
+  // adjacent lanes (bq % 2 = t % 2, row = (t / 2) % 128, chunk bq / 2 = t / 256),
+  // so a wave's 8-byte reads and stores cover contiguous 512-byte runs
+  // without bank conflicts.
+  const int brow = (tid >> 1) & (kBR - 1);
+  const int bq = (tid & 1) | ((tid >> 8) << 1);
   const int boff = (bq >> 1) * (kBR * 8) + brow * 8 + 4 * (bq & 1);
   const int nu = nu0 + brow;
   const float inv_sf = (float)(1.0 / sf[nu]);
@@ -746,7 +749,7 @@ __global__ __launch_bounds__(512, 1) void outer_split(
       yu[e] = (to_f<H>(r0[e]) + to_f<H>(r1[e])) * hfu;
       xf[e] = (to_f<H>(r2[e]) + to_f<H>(r3[e])) * inv_sf;
     }
-    const float* cf = cs + 4 * bq;   // wave-uniform: broadcast reads
+    const float* cf = cs + 4 * bq;   // two addresses per wave: broadcast reads
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const float4 c1 = *reinterpret_cast<const float4*>(cf + (c * 4 + 0) * kBK);
