@@ -56,8 +56,8 @@ struct SplitStore {
   void* Fl = nullptr;   //   lo part
   void* UhT = nullptr;  // mode-major blocks, hi part of the scaled U (outer GEMM)
   void* UlT = nullptr;  //   lo part
-  void* FhT = nullptr;  // mode-major blocks, hi part of the scaled F (outer B build)
-  void* FlT = nullptr;  //   lo part
+  float* XT = nullptr;  // mode-major FP32 blocks of F (outer B build)
+  float* YT = nullptr;  // mode-major FP32 blocks of fY U / 2 (outer B build)
   double* su = nullptr;  // [np] power-of-two column scale of U
   double* sf = nullptr;  // [np] power-of-two column scale of F
   double* umax = nullptr;  // [np] max |U| per column (unscaled)

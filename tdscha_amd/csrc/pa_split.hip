@@ -14,8 +14,11 @@
 // is therefore copied global -> LDS with four 1 KB direct-to-LDS loads
 // (global_load_lds_dwordx4), fully coalesced, without passing through VGPRs.
 //   UA, FA   [Mp/128][np/16] blocks    image-major (weights GEMM, K = modes)
-//   UT, FT   [np/128][Mp/16] blocks    mode-major  (outer GEMM,   K = images)
+//   UT       [np/128][Mp/16] blocks    mode-major  (outer GEMM,   K = images)
 //   YB       [chain][np/128][np/16]    Y1 columns  (weights GEMM B operand)
+// The outer GEMM builds its B operand from F and fY U / 2, which do not change
+// between calls: they are kept as FP32 blocks XT, YT (same element order,
+// 8 KB blocks), as many bytes as a hi/lo pair and no conversions per use.
 //
 // All chains of one pass (up to 3) share a work-group, so every image block
 // is read once per pass.
@@ -155,15 +158,18 @@ __global__ void scales_kernel(const double* __restrict__ umx, const double* __re
 }
 
 // hi/lo split of the images into the image-major and mode-major block
-// layouts (32 x 32 tiles through LDS for the transpose)
+// layouts, and the FP32 mode-major blocks of F and fY U / 2 (32 x 32 tiles
+// through LDS for the transpose)
 template <typename H>
 __global__ void split_images(const double* __restrict__ U, const double* __restrict__ F, long Mp,
                              int np, const double* __restrict__ su,
-                             const double* __restrict__ sf, u16* __restrict__ UAh,
-                             u16* __restrict__ UAl, u16* __restrict__ FAh, u16* __restrict__ FAl,
+                             const double* __restrict__ sf, const double* __restrict__ fy,
+                             u16* __restrict__ UAh, u16* __restrict__ UAl,
+                             u16* __restrict__ FAh, u16* __restrict__ FAl,
                              u16* __restrict__ UTh, u16* __restrict__ UTl,
-                             u16* __restrict__ FTh, u16* __restrict__ FTl) {
-  __shared__ u16 t[4][32][33];
+                             float* __restrict__ XT, float* __restrict__ YT) {
+  __shared__ u16 t[2][32][33];
+  __shared__ float t32[2][32][33];
   const long r0 = (long)blockIdx.x * 32;
   const int c0 = blockIdx.y * 32;
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -171,8 +177,9 @@ __global__ void split_images(const double* __restrict__ U, const double* __restr
   for (int yy = ty; yy < 32; yy += 8) {
     const long r = r0 + yy;
     const int m = c0 + tx;
-    const double u = U[r * np + m] * su[m];
-    const double f = F[r * np + m] * sf[m];
+    const double u0 = U[r * np + m], f0 = F[r * np + m];
+    const double u = u0 * su[m];
+    const double f = f0 * sf[m];
     const u16 uh = from_d<H>(u), fh = from_d<H>(f);
     const u16 ul = from_d<H>(u - (double)to_f<H>(uh));
     const u16 fl = from_d<H>(f - (double)to_f<H>(fh));
@@ -183,8 +190,8 @@ __global__ void split_images(const double* __restrict__ U, const double* __restr
     FAl[a] = fl;
     t[0][yy][tx] = uh;
     t[1][yy][tx] = ul;
-    t[2][yy][tx] = fh;
-    t[3][yy][tx] = fl;
+    t32[0][yy][tx] = (float)f0;
+    t32[1][yy][tx] = (float)(0.5 * fy[m] * u0);
   }
   __syncthreads();
   const long nki = Mp / kBK;
@@ -194,8 +201,8 @@ __global__ void split_images(const double* __restrict__ U, const double* __restr
     const long a = ((long)(m / kBR) * nki + r / kBK) * kBE + blk_off(m % kBR, (int)(r % kBK));
     UTh[a] = t[0][tx][yy];
     UTl[a] = t[1][tx][yy];
-    FTh[a] = t[2][tx][yy];
-    FTl[a] = t[3][tx][yy];
+    XT[a] = t32[0][tx][yy];
+    YT[a] = t32[1][tx][yy];
   }
 }
 
@@ -636,8 +643,8 @@ __global__ void scale_coef_kernel(const double* __restrict__ sb_all, int np, lon
 // ------------------------------------------------------------- outer GEMM
 // Z_c[mu, nu] = sum_k U[k, mu] B_c[k, nu], B_c = c1 F + (c2/2) fY U, over a
 // range of K steps (16 images each, a multiple of 4 steps per range); tile
-// 128 (mu) x 128 (nu).  The A blocks (U^T of the mu tile), the raw U^T / F^T
-// blocks of the nu tile and the step's coefficients arrive by direct-to-LDS
+// 128 (mu) x 128 (nu).  The A blocks (U^T of the mu tile), the FP32 blocks of
+// F and fY U / 2 of the nu tile and the step's coefficients arrive by direct-to-LDS
 // loads three steps ahead in rings of 4 slots (separate arrays, slot index
 // fixed at compile time by a 4-fold unroll).  B_c of step t + 1 is built in
 // FP32 from the raw blocks while the MFMAs of step t run, split hi/lo again
@@ -648,25 +655,23 @@ __global__ void scale_coef_kernel(const double* __restrict__ sb_all, int np, lon
 // are undone in the FP64 reduction.
 template <typename H, int NC>
 __global__ __launch_bounds__(512, 1) void outer_split(
-    const u16* __restrict__ UTh, const u16* __restrict__ UTl, const u16* __restrict__ FTh,
-    const u16* __restrict__ FTl, const float* __restrict__ coefB,
-    const double* __restrict__ su, const double* __restrict__ sf, const double* __restrict__ fy,
-    int np, long Mp, int ksplit, int ntile, int chain0, float* __restrict__ zpart_all,
-    double* __restrict__ fpart_all) {
+    const u16* __restrict__ UTh, const u16* __restrict__ UTl, const float* __restrict__ XT,
+    const float* __restrict__ YT, const float* __restrict__ coefB, int np, long Mp, int ksplit,
+    int ntile, int chain0, float* __restrict__ zpart_all, double* __restrict__ fpart_all) {
   typedef typename Mma<H>::vec vec;
   typedef u16 u16x4v __attribute__((ext_vector_type(4)));
   constexpr int kA = 2 * kBE;                   // A hi/lo
-  constexpr int kR = 4 * kBE;                   // raw U^T h/l, F^T h/l
+  constexpr int kR = 2 * kBE;                   // floats: F, fY U / 2
   constexpr int kB = 2 * NC * kBE;              // B hi/lo per chain
   constexpr int kCoef = NC * 4 * kBK;           // floats of one step's coefficients
   __shared__ __align__(16) u16 sa0[kA];
   __shared__ __align__(16) u16 sa1[kA];
   __shared__ __align__(16) u16 sa2[kA];
   __shared__ __align__(16) u16 sa3[kA];
-  __shared__ __align__(16) u16 sr0[kR];
-  __shared__ __align__(16) u16 sr1[kR];
-  __shared__ __align__(16) u16 sr2[kR];
-  __shared__ __align__(16) u16 sr3[kR];
+  __shared__ __align__(16) float sr0[kR];
+  __shared__ __align__(16) float sr1[kR];
+  __shared__ __align__(16) float sr2[kR];
+  __shared__ __align__(16) float sr3[kR];
   __shared__ __align__(16) u16 sb0[kB];
   __shared__ __align__(16) u16 sb1[kB];
   __shared__ __align__(16) float sc0[kCoef];
@@ -674,7 +679,7 @@ __global__ __launch_bounds__(512, 1) void outer_split(
   __shared__ __align__(16) float sc2[kCoef];
   __shared__ __align__(16) float sc3[kCoef];
   u16* const sA[4] = {sa0, sa1, sa2, sa3};
-  u16* const sR[4] = {sr0, sr1, sr2, sr3};
+  float* const sR[4] = {sr0, sr1, sr2, sr3};
   u16* const sB[2] = {sb0, sb1};
   float* const sC[4] = {sc0, sc1, sc2, sc3};
 
@@ -687,7 +692,10 @@ __global__ __launch_bounds__(512, 1) void outer_split(
   const int tm = tile / ntile, tn = tile % ntile;
   const int mu0 = tm * kBR, nu0 = tn * kBR;
   const bool diag = (tm == tn);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // wave in a scalar register: the copy sources below are then chosen
+  // without branches
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 31, lh = lane >> 5;
   const int wr = wave >> 2, wc = wave & 3;
 
@@ -697,36 +705,29 @@ __global__ __launch_bounds__(512, 1) void outer_split(
   const long n = 4 * ((long)(ks + 1) * nunit / ksplit) - g0;
 
   // builder role: thread t owns images 4 bq .. 4 bq + 3 of row nu0 + brow of
-  // every B block: 8 bytes at element (bq / 2) * 1024 + row * 8 + 4 (bq % 2)
-  // of the chunk-major block.  The two halves of a row's 8-image chunk sit on
+  // every B block: element (bq / 2) * 1024 + row * 8 + 4 (bq % 2) of the
+  // chunk-major block.  The two halves of a row's 8-image chunk sit on
   // adjacent lanes (bq % 2 = t % 2, row = (t / 2) % 128, chunk bq / 2 = t / 256),
-  // so a wave's 8-byte reads and stores cover contiguous 512-byte runs
-  // without bank conflicts.
+  // so a wave's reads and stores cover contiguous runs without bank
+  // conflicts.
   const int brow = (tid >> 1) & (kBR - 1);
   const int bq = (tid & 1) | ((tid >> 8) << 1);
   const int boff = (bq >> 1) * (kBR * 8) + brow * 8 + 4 * (bq & 1);
-  const int nu = nu0 + brow;
-  const float inv_sf = (float)(1.0 / sf[nu]);
-  const float hfu = (float)(0.5 * fy[nu] / su[nu]);
-  const float fyu = (float)(fy[nu] / su[nu]);
 
   const long a_tile = (long)tm * nsub * kBE;
   const long b_tile = (long)tn * nsub * kBE;
-  // the 24 1 KB pieces of a step's A and raw blocks, 3 per wave, and a
-  // 32 NC-byte share of its coefficients: 4 loads per wave
+  // a step's A blocks (2 x 4 KB) and FP32 blocks (2 x 8 KB) in 24 pieces of
+  // 1 KB, 3 per wave, and a 32 NC-byte share of its coefficients: 4 loads per
+  // wave
+  const u16* const UTw = wave < 4 ? UTh : UTl;
   auto issue = [&](long ts, int p) {
     if (ts > n - 1) ts = n - 1;
     const long offa = a_tile + (g0 + ts) * kBE;
     const long offb = b_tile + (g0 + ts) * kBE;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      const int q = wave + 8 * m;               // piece: block q / 4, part q % 4
-      const int blk = q >> 2, part = (q & 3) * 512;
-      const u16* src = blk == 0 ? UTh + offa : blk == 1 ? UTl + offa : blk == 2 ? UTh + offb
-                     : blk == 3 ? UTl + offb : blk == 4 ? FTh + offb : FTl + offb;
-      u16* dst = blk < 2 ? sA[p] + blk * kBE : sR[p] + (blk - 2) * kBE;
-      dma16(src + part + lane * 8, dst + part);
-    }
+    const int part = (wave & 3) * 512;
+    dma16(UTw + offa + part + lane * 8, sA[p] + (wave >> 2) * kBE + part);
+    dma16(XT + offb + wave * 256 + lane * 4, sR[p] + wave * 256);
+    dma16(YT + offb + wave * 256 + lane * 4, sR[p] + kBE + wave * 256);
     const float* cs = coefB + ((g0 + ts) * kMaxChains + chain0) * 4 * kBK + wave * NC * 8 + lane * 4;
     if (lane < 2 * NC) dma16(cs, sC[p] + wave * NC * 8);
   };
@@ -736,19 +737,13 @@ __global__ __launch_bounds__(512, 1) void outer_split(
   for (int c = 0; c < NC; ++c) fl_acc[c] = 0.0;
   // B of one step from its raw blocks and coefficients; on diagonal tiles
   // also the f partial sums (weighted by `live`, 0 for a clamped step)
-  auto build = [&](auto is_diag, const u16* rs, const float* cs, u16* b_slot, double live) {
+  auto build = [&](auto is_diag, const float* rs, const float* cs, u16* b_slot, double live) {
     constexpr bool DIAG = decltype(is_diag)::value;
-    const u16x4v r0 = *reinterpret_cast<const u16x4v*>(rs + boff);
-    const u16x4v r1 = *reinterpret_cast<const u16x4v*>(rs + kBE + boff);
-    const u16x4v r2 = *reinterpret_cast<const u16x4v*>(rs + 2 * kBE + boff);
-    const u16x4v r3 = *reinterpret_cast<const u16x4v*>(rs + 3 * kBE + boff);
-    // xf = F, yu = fY U / 2 (unscaled), shared by the chains
-    float xf[4], yu[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      yu[e] = (to_f<H>(r0[e]) + to_f<H>(r1[e])) * hfu;
-      xf[e] = (to_f<H>(r2[e]) + to_f<H>(r3[e])) * inv_sf;
-    }
+    // xf = F, yu = fY U / 2, shared by the chains
+    const float4 x4 = *reinterpret_cast<const float4*>(rs + boff);
+    const float4 y4 = *reinterpret_cast<const float4*>(rs + kBE + boff);
+    const float xf[4] = {x4.x, x4.y, x4.z, x4.w};
+    const float yu[4] = {y4.x, y4.y, y4.z, y4.w};
     const float* cf = cs + 4 * bq;   // two addresses per wave: broadcast reads
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -759,10 +754,11 @@ __global__ __launch_bounds__(512, 1) void outer_split(
       u16x4v bh, bl;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const float b = c1v[e] * xf[e] + c2v[e] * yu[e];
+        const float b = __builtin_fmaf(c1v[e], xf[e], c2v[e] * yu[e]);
         const u16 hb = from_f<H>(b);
         bh[e] = hb;
-        bl[e] = from_f<H>(b - to_f<H>(hb));
+        // b - hi is exact in FP32; one mixed-precision FMA where the type has it
+        bl[e] = from_f<H>(__builtin_fmaf(to_f<H>(hb), -1.0f, b));
       }
       *reinterpret_cast<u16x4v*>(b_slot + (2 * c) * kBE + boff) = bh;
       *reinterpret_cast<u16x4v*>(b_slot + (2 * c + 1) * kBE + boff) = bl;
@@ -778,7 +774,6 @@ __global__ __launch_bounds__(512, 1) void outer_split(
       }
     }
   };
-  (void)fyu;
 
   v16f acc[NC][2];
 #pragma unroll
@@ -929,8 +924,10 @@ int ensure_split(Ctx* c, int prec) {
   const int np = c->np;
   TDS_REQUIRE(Mp % kBR == 0 && np % kBR == 0, 3, "split path: padded sizes must be multiples of 128");
   const size_t half_bytes = sizeof(u16) * (size_t)Mp * np;
-  void** arr[] = {&st.Uh, &st.Ul, &st.Fh, &st.Fl, &st.UhT, &st.UlT, &st.FhT, &st.FlT};
+  void** arr[] = {&st.Uh, &st.Ul, &st.Fh, &st.Fl, &st.UhT, &st.UlT};
   for (void** p : arr) TDS_CHECK_HIP(hipMalloc(p, half_bytes));
+  TDS_CHECK_HIP(hipMalloc((void**)&st.XT, 2 * half_bytes));
+  TDS_CHECK_HIP(hipMalloc((void**)&st.YT, 2 * half_bytes));
   TDS_CHECK_HIP(hipMalloc((void**)&st.su, sizeof(double) * np));
   TDS_CHECK_HIP(hipMalloc((void**)&st.sf, sizeof(double) * np));
   TDS_CHECK_HIP(hipMalloc((void**)&st.umax, sizeof(double) * np));
@@ -963,12 +960,12 @@ int ensure_split(Ctx* c, int prec) {
   dim3 g((unsigned)(Mp / 32), np / 32);
   if (prec == PREC_FP16X2)
     hipLaunchKernelGGL(split_images<_Float16>, g, dim3(256), 0, s, c->U, c->F, Mp, np, st.su,
-                       st.sf, (u16*)st.Uh, (u16*)st.Ul, (u16*)st.Fh, (u16*)st.Fl, (u16*)st.UhT,
-                       (u16*)st.UlT, (u16*)st.FhT, (u16*)st.FlT);
+                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, (u16*)st.Fh, (u16*)st.Fl,
+                       (u16*)st.UhT, (u16*)st.UlT, st.XT, st.YT);
   else
     hipLaunchKernelGGL(split_images<__bf16>, g, dim3(256), 0, s, c->U, c->F, Mp, np, st.su,
-                       st.sf, (u16*)st.Uh, (u16*)st.Ul, (u16*)st.Fh, (u16*)st.Fl, (u16*)st.UhT,
-                       (u16*)st.UlT, (u16*)st.FhT, (u16*)st.FlT);
+                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, (u16*)st.Fh, (u16*)st.Fl,
+                       (u16*)st.UhT, (u16*)st.UlT, st.XT, st.YT);
   TDS_CHECK_HIP(hipGetLastError());
   TDS_CHECK_HIP(hipStreamSynchronize(s));
   st.ready = true;
@@ -993,8 +990,8 @@ template <typename H, int NC>
 static void launch_outer(Ctx* c, const SplitStore& st, int chain0, hipStream_t s) {
   const int ntile = c->np / kBR;
   hipLaunchKernelGGL((outer_split<H, NC>), dim3((unsigned)(ntile * ntile * c->ksplit_s)),
-                     dim3(512), 0, s, (const u16*)st.UhT, (const u16*)st.UlT, (const u16*)st.FhT,
-                     (const u16*)st.FlT, c->coef32, st.su, st.sf, c->fy, c->np, c->Mp,
+                     dim3(512), 0, s, (const u16*)st.UhT, (const u16*)st.UlT, st.XT, st.YT,
+                     c->coef32, c->np, c->Mp,
                      c->ksplit_s, ntile, chain0, c->zpart32, c->fpart_s);
 }
 
