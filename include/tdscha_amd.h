@@ -164,6 +164,20 @@ int tdscha_qctx_create(tdscha_qctx** out, int n_q, int n_bands, int n_configs, c
                        const double* rho, double T);
 void tdscha_qctx_destroy(tdscha_qctx* ctx);
 
+/*
+ * Point-group images (replaces the sparse q-space symmetry matrices cached by
+ * QSpaceLanczos.prepare_symmetrization, Modules/QSpaceLanczos.py:1471-1633,
+ * and their use per (configuration, symmetry) in the fused kernel,
+ * Modules/tdscha_qspace.jl:496-502).  Symmetry j maps the Bloch components of
+ * source q-point p to q-point q_perm[j * n_q + p] through the n_bands x
+ * n_bands complex block D[((j * n_q + p) * n_bands + nu) * n_bands + mu]
+ * (interleaved re, im).  Expands the context's ensemble on the device into
+ * its n_sym * n_configs images (image j * n_configs + i), normalised by
+ * n_sym * sum(rho); call once, before any run.  Rows of q_perm that are not
+ * permutations are rejected.
+ */
+int tdscha_qctx_symmetrize(tdscha_qctx* ctx, int n_sym, const int* q_perm, const double* D);
+
 /* psi length of the layout built from one q_pair_map (QL:313-343). */
 int tdscha_qspace_psi_size(int n_q, int n_bands, const int* q_pair_map, int* psi_size);
 

@@ -12,7 +12,10 @@ Numpy restatement, complex128, of the reference's q-space path:
     f_Y / f_psi at JQ:680-699), vectorised over configurations;
   * the anharmonic output assembly (QL:718-758) and apply_full_L (QL:870-893);
   * the Hermitian Lanczos with full reorthogonalisation (QL:967-1182,
-    reorthogonalize=True branch).
+    reorthogonalize=True branch);
+  * point-group images: every configuration rotated by the q-space symmetry
+    matrices, x_rot = S_j x (JQ:496-502, matrices built as in QL:1550-1633),
+    normalised by n_syms * N_eff (JQ:621).
 Only the tests use this file.
 
 Pins (tests/test_qspace_oracle.py): on the 2x2x2 grid (the reference's own
@@ -25,7 +28,11 @@ orthonormality of the reorthogonalised basis pin the rest.  On larger grids
 the reference pairs q1 + q2 = q_pert while its products conj(x_q1) x_q2
 carry momentum q2 - q1, so its operator there is not the real-space one; this
 restatement follows the reference (its arithmetic on those grids is pinned
-only through the shared code path and the invariants).
+only through the shared code path and the invariants).  The symmetry
+matrices are pinned directly: the Bloch components of every configuration
+rotated in real space equal the rotated Bloch components (1e-13, all 48
+operations, 2x2x2 and 3x3x3 grids); with them, the 2x2x2 q-space chains equal
+the real-space chains averaged over point group x translations.
 """
 from __future__ import annotations
 
@@ -35,16 +42,37 @@ RY_TO_K = 157887.32400374097
 EPS = 1e-12
 
 
+def rotate_images(Z, perm, D):
+    """Every configuration under every symmetry (JQ:496-502): Z (n_q, N, nb)
+    -> (n_q, n_sym * N, nb), image j * N + i = S_j z_i with
+    (S_j z)[perm[j, p]] = D[j, p] z[p]."""
+    Z = np.asarray(Z, np.complex128)
+    nq, N, nb = Z.shape
+    ns = len(perm)
+    out = np.zeros((nq, ns * N, nb), np.complex128)
+    for j in range(ns):
+        for p in range(nq):
+            out[perm[j, p], j * N:(j + 1) * N] = Z[p] @ D[j, p].T
+    return out
+
+
 class QSystem:
-    """Everything the q-space L reads (reference attribute names)."""
+    """Everything the q-space L reads (reference attribute names).
+    ``symmetries`` = (perm, D) of qspace.q_symmetries, or None (identity)."""
 
     def __init__(self, X_q, Y_q, w_q, valid_modes_q, rho, T, pair_map, unique_pairs, iq_pert,
-                 ignore_harmonic=False, ignore_v3=False, ignore_v4=False):
+                 ignore_harmonic=False, ignore_v3=False, ignore_v4=False, symmetries=None):
+        rho = np.asarray(rho, np.float64)
+        if symmetries is not None:
+            perm, D = symmetries
+            X_q = rotate_images(X_q, perm, D)
+            Y_q = rotate_images(Y_q, perm, D)
+            rho = np.tile(rho, len(perm))      # n_syms * N_eff in the normalisation
         self.X_q = np.asarray(X_q, np.complex128)       # (n_q, N, nb)
         self.Y_q = np.asarray(Y_q, np.complex128)
         self.w_q = np.asarray(w_q, np.float64)           # (nb, n_q)
         self.valid = np.asarray(valid_modes_q, bool)     # (nb, n_q)
-        self.rho = np.asarray(rho, np.float64)
+        self.rho = rho
         self.T = float(T)
         self.n_q, self.N, self.nb = self.X_q.shape
         self.q_pair_map = np.asarray(pair_map)

@@ -56,6 +56,7 @@ def _signatures(L):
                                      _vp, _vp, C.c_double]
     L.tdscha_qctx_destroy.argtypes = [_vp]
     L.tdscha_qctx_destroy.restype = None
+    L.tdscha_qctx_symmetrize.argtypes = [_vp, C.c_int, _vp, _vp]
     L.tdscha_qspace_psi_size.argtypes = [C.c_int, C.c_int, _vp, _ip]
     L.tdscha_qspace_apply_L.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                         C.c_int]

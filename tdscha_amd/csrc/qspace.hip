@@ -2,8 +2,8 @@
 //
 // L application of Modules/QSpaceLanczos.py (apply_full_L, QL:870-893):
 // harmonic part (QL:485-576) plus the perturbed averages of
-// Modules/tdscha_qspace.jl:455-628 (one symmetry, the identity) assembled
-// into the a'/b' blocks (QL:718-758), for a batch of chains that each have
+// Modules/tdscha_qspace.jl:455-628 assembled into the a'/b' blocks
+// (QL:718-758), for a batch of chains that each have
 // their own perturbation q-point (q_pert) and hence their own pair list and
 // psi layout (QL:297-343).  The Hermitian Lanczos with full
 // reorthogonalisation (QL:967-1182) runs on the device for the whole batch,
@@ -23,6 +23,10 @@
 //              (JQ:575-610)
 //   q_finish   fixed-order reduction of the partials, normalisation and
 //              assembly of harmonic + anharmonic parts
+// Point-group images (the per-(configuration, symmetry) rotation
+// x_rot = S_j x of JQ:496-502) are expanded once per context by q_expand:
+// the ensemble then holds n_sym N configurations, normalised by
+// n_sym N_eff (JQ:621), and every kernel above runs unchanged.
 // All sums are fixed-order trees or loops: results do not depend on timing.
 #include <algorithm>
 #include <cmath>
@@ -376,6 +380,41 @@ __global__ void q_finish(const double2* __restrict__ psi, int ld, double2* __res
   }
   po[ea] = oa;
   po[eb] = ob;
+}
+
+// Images of every configuration under every symmetry: image j N + i of
+// q-point perm[j][p] is D[j][p] x_i[p] (JQ:496-502).  Grid
+// (ceil(N / 256), n_sym, n_q), configurations fastest on both sides.
+template <int NB>
+__global__ void q_expand(const double2* __restrict__ X, const double2* __restrict__ Y, int N,
+                         int nq, const int* __restrict__ perm, const double2* __restrict__ D,
+                         double2* __restrict__ XE, double2* __restrict__ YE) {
+  const int p = blockIdx.z, j = blockIdx.y;
+  __shared__ double2 d[NB * NB];
+  for (int e = threadIdx.x; e < NB * NB; e += blockDim.x)
+    d[e] = D[((long)j * nq + p) * NB * NB + e];
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int q = perm[(long)j * nq + p];
+  const long NE = (long)N * gridDim.y;
+  double2 x[NB], y[NB];
+#pragma unroll
+  for (int m = 0; m < NB; ++m) {
+    x[m] = X[((long)p * NB + m) * N + i];
+    y[m] = Y[((long)p * NB + m) * N + i];
+  }
+#pragma unroll
+  for (int v = 0; v < NB; ++v) {
+    double2 sx = make_double2(0.0, 0.0), sy = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int m = 0; m < NB; ++m) {
+      sx = cadd(sx, cmul(d[v * NB + m], x[m]));
+      sy = cadd(sy, cmul(d[v * NB + m], y[m]));
+    }
+    XE[((long)q * NB + v) * NE + (long)j * N + i] = sx;
+    YE[((long)q * NB + v) * NE + (long)j * N + i] = sy;
+  }
 }
 
 // ------------------------------------------------------------ batch setup
@@ -887,6 +926,82 @@ void tdscha_qctx_destroy(tdscha_qctx* ctx) {
     if (x) (void)hipFree(x);
   if (q->stream) (void)hipStreamDestroy(q->stream);
   delete q;
+}
+
+int tdscha_qctx_symmetrize(tdscha_qctx* ctx, int n_sym, const int* q_perm, const double* D) {
+  TDS_REQUIRE(ctx && q_perm && D, 2, "tdscha_qctx_symmetrize: NULL argument");
+  QCtx* q = reinterpret_cast<QCtx*>(ctx);
+  const int nq = q->nq, nb = q->nb, N = q->N;
+  TDS_REQUIRE(n_sym >= 1 && n_sym <= 65535, 2, "tdscha_qctx_symmetrize: n_sym must be in [1, 65535]");
+  TDS_REQUIRE((long)N * n_sym <= 0x7fffffffL, 2,
+              "tdscha_qctx_symmetrize: n_sym * n_configs exceeds 2^31 - 1");
+  for (int j = 0; j < n_sym; ++j) {
+    std::vector<char> seen(nq, 0);
+    for (int p = 0; p < nq; ++p) {
+      const int t = q_perm[(long)j * nq + p];
+      TDS_REQUIRE(t >= 0 && t < nq && !seen[t], 2,
+                  "tdscha_qctx_symmetrize: q_perm row is not a permutation of the q-points");
+      seen[t] = 1;
+    }
+  }
+  const int N2 = N * n_sym;
+  const size_t ne2 = (size_t)nq * nb * N2;
+  std::vector<double> hrho(N), hrho2((size_t)N2);
+  if (hipMemcpy(hrho.data(), q->rho, sizeof(double) * N, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("tdscha_qctx_symmetrize: rho download failed");
+    return -1;
+  }
+  for (int j = 0; j < n_sym; ++j)
+    for (int i = 0; i < N; ++i) hrho2[(size_t)j * N + i] = hrho[i];
+  double2 *XE = nullptr, *YE = nullptr, *dD = nullptr;
+  double* rho2 = nullptr;
+  int* dperm = nullptr;
+  auto release = [&]() {
+    void* p[] = {XE, YE, dD, rho2, dperm};
+    for (void* x : p)
+      if (x) (void)hipFree(x);
+  };
+  bool ok = hipMalloc((void**)&XE, sizeof(double2) * ne2) == hipSuccess &&
+            hipMalloc((void**)&YE, sizeof(double2) * ne2) == hipSuccess &&
+            hipMalloc((void**)&rho2, sizeof(double) * N2) == hipSuccess &&
+            hipMalloc((void**)&dperm, sizeof(int) * (size_t)n_sym * nq) == hipSuccess &&
+            hipMalloc((void**)&dD, sizeof(double2) * (size_t)n_sym * nq * nb * nb) == hipSuccess;
+  if (!ok) {
+    release();
+    set_error("tdscha_qctx_symmetrize: device allocation failed");
+    return -1;
+  }
+  ok = hipMemcpy(rho2, hrho2.data(), sizeof(double) * N2, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(dperm, q_perm, sizeof(int) * (size_t)n_sym * nq, hipMemcpyHostToDevice) ==
+           hipSuccess &&
+       hipMemcpy(dD, D, sizeof(double2) * (size_t)n_sym * nq * nb * nb, hipMemcpyHostToDevice) ==
+           hipSuccess;
+  if (!ok) {
+    release();
+    set_error("tdscha_qctx_symmetrize: upload failed");
+    return -1;
+  }
+  const dim3 g((unsigned)((N + 255) / 256), (unsigned)n_sym, (unsigned)nq);
+  if (nb == 6)
+    hipLaunchKernelGGL(q_expand<6>, g, dim3(256), 0, q->stream, q->X, q->Y, N, nq, dperm, dD, XE, YE);
+  else
+    hipLaunchKernelGGL(q_expand<3>, g, dim3(256), 0, q->stream, q->X, q->Y, N, nq, dperm, dD, XE, YE);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(q->stream) != hipSuccess) {
+    release();
+    set_error("tdscha_qctx_symmetrize: expansion kernel failed");
+    return -1;
+  }
+  (void)hipFree(q->X);
+  (void)hipFree(q->Y);
+  (void)hipFree(q->rho);
+  (void)hipFree(dperm);
+  (void)hipFree(dD);
+  q->X = XE;
+  q->Y = YE;
+  q->rho = rho2;
+  q->N = N2;
+  q->neff *= n_sym;
+  return 0;
 }
 
 int tdscha_qspace_psi_size(int n_q, int n_bands, const int* q_pair_map, int* psi_size) {

@@ -10,8 +10,10 @@ L application and the whole Lanczos run execute in the gfx950 library
 (``tdscha_qctx_*`` / ``tdscha_qspace_*``); ``run_FT_batched`` runs many
 chains with different perturbation q-points in one device-resident batch.
 
-Point-group images are not implemented (the reference's
-``prepare_symmetrization(no_sym=True)``: one symmetry, the identity).
+Point-group images (``init(use_symmetries=True)``, QL:1471-1633): the q-space
+symmetry matrices of ``qspace.q_symmetries`` expand the ensemble on the device
+into its n_sym N images (``tdscha_qctx_symmetrize``), the per-(configuration,
+symmetry) rotation of JQ:496-502.
 """
 from __future__ import annotations
 
@@ -21,7 +23,7 @@ import numpy as np
 
 from . import _lib
 from .lanczos import continued_fraction
-from .qspace import BlochModes, bloch_ensemble, pair_map as _pair_map
+from .qspace import BlochModes, bloch_ensemble, pair_map as _pair_map, q_symmetries
 
 EPSILON = 1e-12
 
@@ -41,11 +43,26 @@ class QSpaceEngine:
         v = np.ascontiguousarray(valid_modes_q, np.int32)
         r = np.ascontiguousarray(rho, np.float64)
         h = C.c_void_p()
+        self.n_syms = 1
         _lib.check(_lib.lib().tdscha_qctx_create(C.byref(h), self.n_q, self.n_bands, self.N,
                                                  _p(X_q.view(np.float64)),
                                                  _p(Y_q.view(np.float64)), _p(w), _p(v), _p(r),
                                                  float(T)), "tdscha_qctx_create")
         self.ctx = h
+
+    def symmetrize(self, perm, D):
+        """Expand the ensemble into its images under the symmetries (perm, D)
+        of qspace.q_symmetries; once, before any run."""
+        perm = np.ascontiguousarray(perm, np.int32)
+        D = np.ascontiguousarray(D, np.complex128)
+        ns = perm.shape[0]
+        if perm.shape != (ns, self.n_q) or D.shape != (ns, self.n_q, self.n_bands, self.n_bands):
+            raise ValueError("symmetry tables do not match the ensemble's q-points and bands")
+        _lib.check(_lib.lib().tdscha_qctx_symmetrize(self.ctx, ns, _p(perm),
+                                                     _p(D.view(np.float64))),
+                   "tdscha_qctx_symmetrize")
+        self.N *= ns
+        self.n_syms = ns
 
     def psi_size(self, pm):
         n = C.c_int()
@@ -143,6 +160,8 @@ class QSpaceLanczos(object):
         self.q_pair_map = None
         self.unique_pairs = None
         self.engine = None
+        self.initialized = False
+        self.n_syms_qspace = 1
         if ensemble is None:
             return
         if X_q is None:
@@ -161,14 +180,27 @@ class QSpaceLanczos(object):
         self.rho = np.asarray(ensemble.rho, np.float64)
         self.N = len(self.rho)
 
-    def init(self, use_symmetries=False):
-        if use_symmetries:
-            raise NotImplementedError("q-space point-group images are not implemented; use "
-                                      "use_symmetries=False (the identity only)")
+    def init(self, use_symmetries=True):
+        """QL:1658-1662."""
+        self.prepare_symmetrization(no_sym=not use_symmetries)
+
+    def prepare_symmetrization(self, no_sym=False, verbose=False, symmetries=None):
+        """QL:1471-1633: upload the ensemble and, unless ``no_sym``, expand it
+        into its point-group images (``symmetries`` = (perm, D), default the
+        48 operations of O_h built by qspace.q_symmetries)."""
         if self.engine is not None:
             self.engine.close()
         self.engine = QSpaceEngine(self.X_q, self.Y_q, self.w_q, self.valid_modes_q, self.rho,
                                    self.T)
+        self.n_syms_qspace = 1
+        if not no_sym:
+            perm, D = q_symmetries(self.modes) if symmetries is None else symmetries
+            self.engine.symmetrize(perm, D)
+            self.n_syms_qspace = len(perm)
+            if verbose:
+                print("Q-space: %d point-group images of %d configurations"
+                      % (self.n_syms_qspace, self.N))
+        self.initialized = True
 
     # ---------------------------------------------------------- layout
     def build_q_pair_map(self, iq_pert):

@@ -127,3 +127,66 @@ def bloch_ensemble(ens, modes=None):
     ut = ens.X @ ens.pols.T
     ft = ens.Y @ ens.pols.T
     return modes, modes.transform(ut), modes.transform(ft)
+
+
+def q_symmetries(modes, rotations=None):
+    """Point-group images in the Bloch basis (the q-space symmetry matrices of
+    QSpaceLanczos._build_qspace_symmetries, Modules/QSpaceLanczos.py:1550-1633).
+
+    For a rotation R about the cation at the origin (t = 0), an atom k of the
+    primitive cell at tau_k goes to R tau_k = tau_{irt[k]} + M L_k (M the
+    primitive lattice vectors as columns, L_k integer), and cell n to
+    R_f n + L_k with R_f = M^-1 R M.  The Bloch components of the rotated
+    mass-scaled displacement are then, for every source q-point p,
+
+        x'[q] = D[p] x[p],  m_q = R_f^-T m_p (mod L),
+        D[p]_{nu mu} = sum_k eps_q(irt[k])^H_nu R exp(-2 pi i m_q . L_k / L) eps_p(k)_mu
+
+    with eps the Bloch eigenvectors of ``modes``.  Returns ``perm``
+    (n_sym, n_q) int32, perm[j, p] = q, and ``D`` (n_sym, n_q, nb, nb)
+    complex.  ``rotations`` defaults to the 48 elements of O_h (identity
+    first)."""
+    from .symmetry import cubic_rotations
+    model = modes.model
+    L, nq, nb = modes.L, modes.n_q, modes.n_bands
+    if rotations is None:
+        rotations = cubic_rotations()
+    M = model.cell.T
+    Minv = np.linalg.inv(M)
+    tau = model.positions[:2]
+    grid = modes.grid
+    perms, Ds = [], []
+    for R in rotations:
+        Rf = Minv @ R @ M
+        Rfi = np.round(Rf)
+        if np.abs(Rf - Rfi).max() > 1e-8:
+            raise ValueError("rotation is not a symmetry of the lattice")
+        RfTinv = np.round(np.linalg.inv(Rfi).T).astype(np.int64)
+        irt = np.zeros(2, np.int64)
+        Lk = np.zeros((2, 3), np.int64)
+        for k in range(2):
+            r = R @ tau[k]
+            for kp in range(2):
+                d = Minv @ (r - tau[kp])
+                if np.abs(d - np.round(d)).max() < 1e-8 and model.species[kp] == model.species[k]:
+                    irt[k] = kp
+                    Lk[k] = np.round(d).astype(np.int64)
+                    break
+            else:
+                raise ValueError("rotation does not map the primitive cell onto itself")
+        perm = np.zeros(nq, np.int32)
+        D = np.zeros((nq, nb, nb), np.complex128)
+        for p in range(nq):
+            mq = np.mod(RfTinv @ grid[p], L)
+            q = q_index(mq, L)
+            perm[p] = q
+            P = np.zeros((nb, nb), np.complex128)
+            for k in range(2):
+                ph = np.exp(-2j * np.pi * float(mq @ Lk[k]) / L)
+                P[3 * irt[k]:3 * irt[k] + 3, 3 * k:3 * k + 3] = ph * R
+            D[p] = modes.eps[q].conj().T @ P @ modes.eps[p]
+        if len(set(perm.tolist())) != nq:
+            raise ValueError("q-point map is not a permutation")
+        perms.append(perm)
+        Ds.append(D)
+    return np.array(perms, dtype=np.int32), np.array(Ds)

@@ -134,3 +134,59 @@ def test_autotune_picks_valid_tiles_and_keeps_results():
     for x, y in zip(before, after):
         assert _rel(x, y) < 1e-12
     eng.close()
+
+
+@pytest.mark.parametrize("L,N,iqs", [(2, 30, [0, 3, 7]), (4, 8, [0, 1, 21, 42])])
+def test_symmetrized_apply_L_matches_oracle(L, N, iqs):
+    """Point-group images expanded on the device (48 per configuration) against
+    the oracle's rotated ensemble."""
+    from tdscha_amd.qspace import q_symmetries
+    ens, modes, Xq, Yq, eng = _setup(L, N)
+    sym = q_symmetries(modes)
+    eng.symmetrize(*sym)
+    assert eng.N == 48 * N
+    rng = np.random.default_rng(17)
+    pms, psis, refs = [], [], []
+    for iq in iqs:
+        qs = _oracle(ens, modes, Xq, Yq, iq, symmetries=sym)
+        psi = rng.normal(size=qs.psi_size) + 1j * rng.normal(size=qs.psi_size)
+        pms.append(qs.q_pair_map)
+        psis.append(psi)
+        refs.append(qs.apply_full_L(psi))
+    outs = eng.apply_L(iqs, pms, psis)
+    for iq, o, r in zip(iqs, outs, refs):
+        err = _rel(o, r)
+        print("L=%d iq=%d symmetrised rel err %.2e" % (L, iq, err))
+        assert err < 1e-12
+    eng.close()
+
+
+def test_symmetrized_lanczos_and_mirror_match_oracle():
+    """QSpaceLanczos.init(use_symmetries=True) + run_FT against the oracle's
+    Hermitian Lanczos on the rotated ensemble; a bad q-point table is
+    rejected."""
+    from tdscha_amd.qlanczos import QSpaceLanczos
+    from tdscha_amd.qspace import q_symmetries
+    from tdscha_amd.rocksalt import Ensemble, RockSalt
+    ens = Ensemble(RockSalt(L=4), 300.0, 10, seed=8)
+    lanc = QSpaceLanczos(ens)
+    lanc.init(use_symmetries=True)
+    assert lanc.n_syms_qspace == 48 and lanc.engine.N == 480
+    sym = q_symmetries(lanc.modes)
+    for iq, band in ((0, 4), (9, 1)):
+        lanc.prepare_mode_q(iq, band)
+        a, b = lanc.run_FT(12)
+        qs = _oracle(ens, lanc.modes, lanc.X_q, lanc.Y_q, iq, symmetries=sym)
+        psi = np.zeros(qs.psi_size, np.complex128)
+        psi[band] = 1.0
+        ao, bo, _, _ = QO.hermitian_lanczos(qs.apply_full_L, psi, qs.mask(), 12, mgs=False)
+        print("iq %d a %s / %s" % (iq, a[:3], ao[:3]))
+        assert len(a) == len(ao)
+        np.testing.assert_allclose(a, ao, rtol=1e-10)
+        np.testing.assert_allclose(b, bo, rtol=1e-10)
+    perm, D = sym
+    bad = perm[:2].copy()
+    bad[1, 0] = bad[1, 1]
+    with pytest.raises(RuntimeError, match="not a permutation"):
+        lanc.engine.symmetrize(bad, D[:2])
+    lanc.close()

@@ -114,3 +114,68 @@ def test_mgs_and_cgs2_reorthogonalisation_agree():
     a2, b2, _, _ = QO.hermitian_lanczos(qs.apply_full_L, psi, qs.mask(), 30, mgs=False)
     np.testing.assert_allclose(a1, a2, rtol=1e-10)
     np.testing.assert_allclose(b1, b2, rtol=1e-10)
+
+
+def _space_group(ens):
+    """Mode-space matrices of the 48 point-group operations (about the cation
+    at the origin) times the supercell translations."""
+    m = ens.model
+    ops = []
+    for R in SY.cubic_rotations():
+        for t in m.translations:
+            irt = SY.atom_permutation(m.positions, m.lattice, R, m.positions[0],
+                                      np.asarray(t, float))
+            ops.append(ens.pols.T @ SY.cartesian_operator(irt, R) @ ens.pols)
+    return np.array(ops)
+
+
+@pytest.mark.parametrize("L", [2, 3])
+def test_q_symmetries_rotate_bloch_components(L):
+    """The Bloch components of a configuration rotated in real space equal the
+    q-space symmetry matrices applied to its Bloch components, for all 48
+    operations (on the 3x3x3 grid most q-points are not their own inverse,
+    so the phases are complex)."""
+    from tdscha_amd.qspace import q_symmetries
+    model = RockSalt(L=L)
+    modes = BlochModes(model)
+    perm, D = q_symmetries(modes)
+    assert perm.shape == (48, modes.n_q) and np.array_equal(perm[0], np.arange(modes.n_q))
+    assert np.abs(D[0] - np.eye(6)[None]).max() < 1e-13
+    rng = np.random.default_rng(L)
+    z = rng.normal(size=(3, 3 * model.nat))
+    X = modes.transform(z)
+    rot = QO.rotate_images(X, perm, D)
+    for j, R in enumerate(SY.cubic_rotations()):
+        irt = SY.atom_permutation(model.positions, model.lattice, R, model.positions[0],
+                                  np.zeros(3))
+        Xr = modes.transform(z @ SY.cartesian_operator(irt, R).T)
+        err = np.abs(rot[:, 3 * j:3 * j + 3] - Xr).max() / np.abs(Xr).max()
+        assert err < 1e-13, (j, err)
+    for j in range(48):
+        for p in range(modes.n_q):
+            assert np.abs(D[j, p] @ D[j, p].conj().T - np.eye(6)).max() < 1e-13
+
+
+@pytest.mark.parametrize("iq,band", [(0, 3), (1, 0), (6, 4)])
+def test_chain_with_point_group_equals_real_space_space_group(iq, band):
+    """2x2x2 grid: the q-space chain with the 48 point-group images equals the
+    real-space chain averaged over point group x translations (384 images),
+    at Gamma and at q_pert != Gamma."""
+    from tdscha_amd.qspace import q_symmetries
+    ens, modes, Xq, Yq = _system(N=12)
+    sym = q_symmetries(modes)
+    qs = _qsys(modes, Xq, Yq, ens, iq, symmetries=sym)
+    assert qs.N == 48 * 12
+    psi_q = np.zeros(qs.psi_size, np.complex128)
+    psi_q[band] = 1.0
+    steps = 8
+    aq, bq, _, _ = QO.hermitian_lanczos(qs.apply_full_L, psi_q, qs.mask(), steps)
+    n = len(ens.w)
+    orc = LO.OracleSystem(ens.X, ens.Y, ens.w, ens.rho, ens.T, _space_group(ens))
+    mv = lambda v: orc.apply_full_L(v.real) + 1j * orc.apply_full_L(v.imag)
+    psi_r = np.zeros(LO.len_psi(n), np.complex128)
+    psi_r[:n] = ens.pols.T @ modes.basis()[:, iq * modes.n_bands + band]
+    ar, br, _, _ = QO.hermitian_lanczos(mv, psi_r, LO.mask_dot_wigner(n), steps)
+    print("a q-space", aq[:3], "real", ar[:3])
+    np.testing.assert_allclose(aq, ar, rtol=1e-9)
+    np.testing.assert_allclose(bq, br, rtol=1e-9)
