@@ -52,8 +52,7 @@ enum Precision { PREC_FP64 = 0, PREC_FP16X2 = 1, PREC_BF16X2 = 2 };
 struct SplitStore {
   void* Uh = nullptr;   // image-major blocks, hi part of the scaled U (weights A operand)
   void* Ul = nullptr;   //   lo part
-  void* Fh = nullptr;   // image-major blocks, hi part of the scaled F (weights epilogue)
-  void* Fl = nullptr;   //   lo part
+  float* FA = nullptr;  // image-major FP32 blocks of F (weights epilogue)
   void* UhT = nullptr;  // mode-major blocks, hi part of the scaled U (outer GEMM)
   void* UlT = nullptr;  //   lo part
   float* XT = nullptr;  // mode-major FP32 blocks of F (outer B build)

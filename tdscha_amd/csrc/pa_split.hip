@@ -13,12 +13,14 @@
 // the MFMA fragment reads want (16-byte chunks XOR-swizzled by row).  A block
 // is therefore copied global -> LDS with four 1 KB direct-to-LDS loads
 // (global_load_lds_dwordx4), fully coalesced, without passing through VGPRs.
-//   UA, FA   [Mp/128][np/16] blocks    image-major (weights GEMM, K = modes)
+//   UA       [Mp/128][np/16] blocks    image-major (weights GEMM, K = modes)
 //   UT       [np/128][Mp/16] blocks    mode-major  (outer GEMM,   K = images)
 //   YB       [chain][np/128][np/16]    Y1 columns  (weights GEMM B operand)
-// The outer GEMM builds its B operand from F and fY U / 2, which do not change
-// between calls: they are kept as FP32 blocks XT, YT (same element order,
-// 8 KB blocks), as many bytes as a hi/lo pair and no conversions per use.
+// The outer GEMM builds its B operand from F and fY U / 2, and the weights
+// epilogue reads F; none of them changes between calls, so they are kept as
+// FP32 blocks XT, YT (mode-major) and FA (image-major), same element order
+// as the 16-bit blocks (8 KB blocks): as many bytes as a hi/lo pair and no
+// conversions per use.
 //
 // All chains of one pass (up to 3) share a work-group, so every image block
 // is read once per pass.
@@ -165,7 +167,7 @@ __global__ void split_images(const double* __restrict__ U, const double* __restr
                              int np, const double* __restrict__ su,
                              const double* __restrict__ sf, const double* __restrict__ fy,
                              u16* __restrict__ UAh, u16* __restrict__ UAl,
-                             u16* __restrict__ FAh, u16* __restrict__ FAl,
+                             float* __restrict__ FA,
                              u16* __restrict__ UTh, u16* __restrict__ UTl,
                              float* __restrict__ XT, float* __restrict__ YT) {
   __shared__ u16 t[2][32][33];
@@ -179,15 +181,12 @@ __global__ void split_images(const double* __restrict__ U, const double* __restr
     const int m = c0 + tx;
     const double u0 = U[r * np + m], f0 = F[r * np + m];
     const double u = u0 * su[m];
-    const double f = f0 * sf[m];
-    const u16 uh = from_d<H>(u), fh = from_d<H>(f);
+    const u16 uh = from_d<H>(u);
     const u16 ul = from_d<H>(u - (double)to_f<H>(uh));
-    const u16 fl = from_d<H>(f - (double)to_f<H>(fh));
     const long a = ((r / kBR) * nkc + m / kBK) * kBE + blk_off((int)(r % kBR), m % kBK);
     UAh[a] = uh;
     UAl[a] = ul;
-    FAh[a] = fh;
-    FAl[a] = fl;
+    FA[a] = (float)f0;
     t[0][yy][tx] = uh;
     t[1][yy][tx] = ul;
     t32[0][yy][tx] = (float)f0;
@@ -298,8 +297,8 @@ using ic = std::integral_constant<int, K>;
 
 template <typename H, int NC>
 __global__ __launch_bounds__(256, 1) void weights_split(
-    const u16* __restrict__ UAh, const u16* __restrict__ UAl, const u16* __restrict__ FAh,
-    const u16* __restrict__ FAl, const u16* __restrict__ YBh, const u16* __restrict__ YBl,
+    const u16* __restrict__ UAh, const u16* __restrict__ UAl, const float* __restrict__ FA,
+    const u16* __restrict__ YBh, const u16* __restrict__ YBl,
     const float* __restrict__ colw, const float* __restrict__ epw, int np, long Mp, int chain0,
     int nrt, double* __restrict__ part) {
   typedef typename Mma<H>::vec vec;
@@ -439,17 +438,20 @@ __global__ __launch_bounds__(256, 1) void weights_split(
     // then the four quarters are summed in a fixed order.  The image loads of
     // row group i + 1 (16 loads of 8 bytes per lane) are issued before row
     // group i is multiplied, so their latency overlaps the arithmetic.
-    u16x4 ldb[2][4][4];
-    auto load_rows = [&](int i, u16x4 (&ld)[4][4]) {
+    struct Rows {
+      u16x4 u[4][2];    // U hi / lo of the lane's 4 columns, per column quarter
+      float4 f[4];      // F (FP32)
+    };
+    Rows ldb[2];
+    auto load_rows = [&](int i, Rows& ld) {
       const int row = 32 * i + lr;
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int col = 32 * wave + 8 * q + 4 * lh;
         const long a = ((long)rt * nkc + (c0 + col) / kBK) * kBE + blk_off(row, col % kBK);
-        ld[q][0] = *reinterpret_cast<const u16x4*>(UAh + a);
-        ld[q][1] = *reinterpret_cast<const u16x4*>(UAl + a);
-        ld[q][2] = *reinterpret_cast<const u16x4*>(FAh + a);
-        ld[q][3] = *reinterpret_cast<const u16x4*>(FAl + a);
+        ld.u[q][0] = *reinterpret_cast<const u16x4*>(UAh + a);
+        ld.u[q][1] = *reinterpret_cast<const u16x4*>(UAl + a);
+        ld.f[q] = *reinterpret_cast<const float4*>(FA + a);
       }
     };
     load_rows(0, ldb[0]);
@@ -462,21 +464,19 @@ __global__ __launch_bounds__(256, 1) void weights_split(
 #pragma unroll
         for (int c = 0; c < NC; ++c) p[c][0] = p[c][1] = p[c][2] = p[c][3] = 0.0;
         if (i < 3) load_rows(i + 1, ldb[(i + 1) & 1]);
-        u16x4 (&ld)[4][4] = ldb[i & 1];
+        const Rows& ld = ldb[i & 1];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int col = 32 * wave + 8 * q + 4 * lh;   // tile column of t = 0
           const float4 isu = *reinterpret_cast<const float4*>(cw + 0 * kBR + col);
-          const float4 isf = *reinterpret_cast<const float4*>(cw + 1 * kBR + col);
           const float4 fps = *reinterpret_cast<const float4*>(cw + 2 * kBR + col);
           const float su4[4] = {isu.x, isu.y, isu.z, isu.w};
-          const float sf4[4] = {isf.x, isf.y, isf.z, isf.w};
           const float fp4[4] = {fps.x, fps.y, fps.z, fps.w};
-          float u[4], f[4], g[4];
+          const float f[4] = {ld.f[q].x, ld.f[q].y, ld.f[q].z, ld.f[q].w};
+          float u[4], g[4];
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
-            u[t] = (to_f<H>(ld[q][0][t]) + to_f<H>(ld[q][1][t])) * su4[t];
-            f[t] = (to_f<H>(ld[q][2][t]) + to_f<H>(ld[q][3][t])) * sf4[t];
+            u[t] = (to_f<H>(ld.u[q][0][t]) + to_f<H>(ld.u[q][1][t])) * su4[t];
             g[t] = f[t] * fp4[t];
           }
 #pragma unroll
@@ -924,8 +924,9 @@ int ensure_split(Ctx* c, int prec) {
   const int np = c->np;
   TDS_REQUIRE(Mp % kBR == 0 && np % kBR == 0, 3, "split path: padded sizes must be multiples of 128");
   const size_t half_bytes = sizeof(u16) * (size_t)Mp * np;
-  void** arr[] = {&st.Uh, &st.Ul, &st.Fh, &st.Fl, &st.UhT, &st.UlT};
+  void** arr[] = {&st.Uh, &st.Ul, &st.UhT, &st.UlT};
   for (void** p : arr) TDS_CHECK_HIP(hipMalloc(p, half_bytes));
+  TDS_CHECK_HIP(hipMalloc((void**)&st.FA, 2 * half_bytes));
   TDS_CHECK_HIP(hipMalloc((void**)&st.XT, 2 * half_bytes));
   TDS_CHECK_HIP(hipMalloc((void**)&st.YT, 2 * half_bytes));
   TDS_CHECK_HIP(hipMalloc((void**)&st.su, sizeof(double) * np));
@@ -960,11 +961,11 @@ int ensure_split(Ctx* c, int prec) {
   dim3 g((unsigned)(Mp / 32), np / 32);
   if (prec == PREC_FP16X2)
     hipLaunchKernelGGL(split_images<_Float16>, g, dim3(256), 0, s, c->U, c->F, Mp, np, st.su,
-                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, (u16*)st.Fh, (u16*)st.Fl,
+                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, st.FA,
                        (u16*)st.UhT, (u16*)st.UlT, st.XT, st.YT);
   else
     hipLaunchKernelGGL(split_images<__bf16>, g, dim3(256), 0, s, c->U, c->F, Mp, np, st.su,
-                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, (u16*)st.Fh, (u16*)st.Fl,
+                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, st.FA,
                        (u16*)st.UhT, (u16*)st.UlT, st.XT, st.YT);
   TDS_CHECK_HIP(hipGetLastError());
   TDS_CHECK_HIP(hipStreamSynchronize(s));
@@ -981,7 +982,7 @@ static void launch_weights(Ctx* c, const SplitStore& st, int chain0, hipStream_t
   const int per_xcd_tiles = (nrt + 7) / 8 * nct;
   const int nper = per_xcd_tiles < 32 ? per_xcd_tiles : 32;
   hipLaunchKernelGGL((weights_split<H, NC>), dim3((unsigned)(8 * nper)), dim3(256), 0, s,
-                     (const u16*)st.Uh, (const u16*)st.Ul, (const u16*)st.Fh, (const u16*)st.Fl,
+                     (const u16*)st.Uh, (const u16*)st.Ul, st.FA,
                      (const u16*)c->Y1h, (const u16*)c->Y1l, st.colw, c->epi, np, c->Mp, chain0,
                      nrt, c->wpart);
 }
