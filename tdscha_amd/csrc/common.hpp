@@ -53,6 +53,7 @@ struct SplitStore {
   void* Uh = nullptr;   // image-major blocks, hi part of the scaled U (weights A operand)
   void* Ul = nullptr;   //   lo part
   float* FA = nullptr;  // image-major FP32 blocks of F (weights epilogue)
+  float* UA32 = nullptr;  // image-major FP32 blocks of U (weights epilogue)
   void* UhT = nullptr;  // mode-major blocks, hi part of the scaled U (outer GEMM)
   void* UlT = nullptr;  //   lo part
   float* XT = nullptr;  // mode-major FP32 blocks of F (outer B build)

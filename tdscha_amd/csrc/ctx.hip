@@ -259,7 +259,7 @@ void tdscha_ctx_destroy(tdscha_ctx* ctx) {
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& s : c->split) {
-    void* sp[] = {s.Uh, s.Ul, s.FA, s.UhT, s.UlT, s.XT, s.YT, s.su, s.sf, s.umax, s.fmax, s.colw};
+    void* sp[] = {s.Uh, s.Ul, s.FA, s.UA32, s.UhT, s.UlT, s.XT, s.YT, s.su, s.sf, s.umax, s.fmax, s.colw};
     for (void* p : sp)
       if (p) hipFree(p);
   }

@@ -167,7 +167,7 @@ __global__ void split_images(const double* __restrict__ U, const double* __restr
                              int np, const double* __restrict__ su,
                              const double* __restrict__ sf, const double* __restrict__ fy,
                              u16* __restrict__ UAh, u16* __restrict__ UAl,
-                             float* __restrict__ FA,
+                             float* __restrict__ FA, float* __restrict__ UA32,
                              u16* __restrict__ UTh, u16* __restrict__ UTl,
                              float* __restrict__ XT, float* __restrict__ YT) {
   __shared__ u16 t[2][32][33];
@@ -187,6 +187,7 @@ __global__ void split_images(const double* __restrict__ U, const double* __restr
     UAh[a] = uh;
     UAl[a] = ul;
     FA[a] = (float)f0;
+    UA32[a] = (float)u0;
     t[0][yy][tx] = uh;
     t[1][yy][tx] = ul;
     t32[0][yy][tx] = (float)f0;
@@ -298,6 +299,7 @@ using ic = std::integral_constant<int, K>;
 template <typename H, int NC>
 __global__ __launch_bounds__(256, 1) void weights_split(
     const u16* __restrict__ UAh, const u16* __restrict__ UAl, const float* __restrict__ FA,
+    const float* __restrict__ UA32,
     const u16* __restrict__ YBh, const u16* __restrict__ YBl,
     const float* __restrict__ colw, const float* __restrict__ epw, int np, long Mp, int chain0,
     int nrt, double* __restrict__ part) {
@@ -439,7 +441,7 @@ __global__ __launch_bounds__(256, 1) void weights_split(
     // row group i + 1 (16 loads of 8 bytes per lane) are issued before row
     // group i is multiplied, so their latency overlaps the arithmetic.
     struct Rows {
-      u16x4 u[4][2];    // U hi / lo of the lane's 4 columns, per column quarter
+      float4 u[4];      // U of the lane's 4 columns, per column quarter (FP32)
       float4 f[4];      // F (FP32)
     };
     Rows ldb[2];
@@ -449,8 +451,7 @@ __global__ __launch_bounds__(256, 1) void weights_split(
       for (int q = 0; q < 4; ++q) {
         const int col = 32 * wave + 8 * q + 4 * lh;
         const long a = ((long)rt * nkc + (c0 + col) / kBK) * kBE + blk_off(row, col % kBK);
-        ld.u[q][0] = *reinterpret_cast<const u16x4*>(UAh + a);
-        ld.u[q][1] = *reinterpret_cast<const u16x4*>(UAl + a);
+        ld.u[q] = *reinterpret_cast<const float4*>(UA32 + a);
         ld.f[q] = *reinterpret_cast<const float4*>(FA + a);
       }
     };
@@ -460,50 +461,46 @@ __global__ __launch_bounds__(256, 1) void weights_split(
 #pragma unroll
       for (int ii = 0; ii < 2; ++ii) {
         const int i = 2 * half + ii;
-        double p[NC][4];
+        // FP32 over the lane's 16 columns, FP64 beyond
+        float p32[NC][4];
 #pragma unroll
-        for (int c = 0; c < NC; ++c) p[c][0] = p[c][1] = p[c][2] = p[c][3] = 0.0;
+        for (int c = 0; c < NC; ++c) p32[c][0] = p32[c][1] = p32[c][2] = p32[c][3] = 0.f;
         if (i < 3) load_rows(i + 1, ldb[(i + 1) & 1]);
         const Rows& ld = ldb[i & 1];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int col = 32 * wave + 8 * q + 4 * lh;   // tile column of t = 0
-          const float4 isu = *reinterpret_cast<const float4*>(cw + 0 * kBR + col);
           const float4 fps = *reinterpret_cast<const float4*>(cw + 2 * kBR + col);
-          const float su4[4] = {isu.x, isu.y, isu.z, isu.w};
           const float fp4[4] = {fps.x, fps.y, fps.z, fps.w};
           const float f[4] = {ld.f[q].x, ld.f[q].y, ld.f[q].z, ld.f[q].w};
-          float u[4], g[4];
+          const float u[4] = {ld.u[q].x, ld.u[q].y, ld.u[q].z, ld.u[q].w};
+          float g[4];
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            u[t] = (to_f<H>(ld.u[q][0][t]) + to_f<H>(ld.u[q][1][t])) * su4[t];
-            g[t] = f[t] * fp4[t];
-          }
+          for (int t = 0; t < 4; ++t) g[t] = f[t] * fp4[t];
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
             const float4 fr = *reinterpret_cast<const float4*>(cw + (3 + 2 * c) * kBR + col);
             const float4 rr = *reinterpret_cast<const float4*>(cw + (4 + 2 * c) * kBR + col);
             const float fr4[4] = {fr.x, fr.y, fr.z, fr.w};
             const float rr4[4] = {rr.x, rr.y, rr.z, rr.w};
-            float q0 = 0.f, q1 = 0.f, q2 = 0.f, q3 = 0.f;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
               const float v = acc[c][i][4 * q + t];
-              q0 += v * u[t];
-              q1 += v * g[t];
-              q2 += u[t] * fr4[t];
-              q3 += f[t] * rr4[t];
+              p32[c][0] += v * u[t];
+              p32[c][1] += v * g[t];
+              p32[c][2] += u[t] * fr4[t];
+              p32[c][3] += f[t] * rr4[t];
             }
-            p[c][0] += (double)q0;
-            p[c][1] += (double)q1;
-            p[c][2] += (double)q2;
-            p[c][3] += (double)q3;
           }
         }
+        double p[NC][4];
 #pragma unroll
         for (int c = 0; c < NC; ++c)
 #pragma unroll
-          for (int qq = 0; qq < 4; ++qq) p[c][qq] += __shfl_xor(p[c][qq], 32);
+          for (int qq = 0; qq < 4; ++qq) {
+            p[c][qq] = (double)p32[c][qq];
+            p[c][qq] += __shfl_xor(p[c][qq], 32);
+          }
         if (lh == 0) {
 #pragma unroll
           for (int c = 0; c < NC; ++c)
@@ -927,6 +924,7 @@ int ensure_split(Ctx* c, int prec) {
   void** arr[] = {&st.Uh, &st.Ul, &st.UhT, &st.UlT};
   for (void** p : arr) TDS_CHECK_HIP(hipMalloc(p, half_bytes));
   TDS_CHECK_HIP(hipMalloc((void**)&st.FA, 2 * half_bytes));
+  TDS_CHECK_HIP(hipMalloc((void**)&st.UA32, 2 * half_bytes));
   TDS_CHECK_HIP(hipMalloc((void**)&st.XT, 2 * half_bytes));
   TDS_CHECK_HIP(hipMalloc((void**)&st.YT, 2 * half_bytes));
   TDS_CHECK_HIP(hipMalloc((void**)&st.su, sizeof(double) * np));
@@ -961,11 +959,11 @@ int ensure_split(Ctx* c, int prec) {
   dim3 g((unsigned)(Mp / 32), np / 32);
   if (prec == PREC_FP16X2)
     hipLaunchKernelGGL(split_images<_Float16>, g, dim3(256), 0, s, c->U, c->F, Mp, np, st.su,
-                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, st.FA,
+                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, st.FA, st.UA32,
                        (u16*)st.UhT, (u16*)st.UlT, st.XT, st.YT);
   else
     hipLaunchKernelGGL(split_images<__bf16>, g, dim3(256), 0, s, c->U, c->F, Mp, np, st.su,
-                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, st.FA,
+                       st.sf, c->fy, (u16*)st.Uh, (u16*)st.Ul, st.FA, st.UA32,
                        (u16*)st.UhT, (u16*)st.UlT, st.XT, st.YT);
   TDS_CHECK_HIP(hipGetLastError());
   TDS_CHECK_HIP(hipStreamSynchronize(s));
@@ -982,7 +980,7 @@ static void launch_weights(Ctx* c, const SplitStore& st, int chain0, hipStream_t
   const int per_xcd_tiles = (nrt + 7) / 8 * nct;
   const int nper = per_xcd_tiles < 32 ? per_xcd_tiles : 32;
   hipLaunchKernelGGL((weights_split<H, NC>), dim3((unsigned)(8 * nper)), dim3(256), 0, s,
-                     (const u16*)st.Uh, (const u16*)st.Ul, st.FA,
+                     (const u16*)st.Uh, (const u16*)st.Ul, st.FA, st.UA32,
                      (const u16*)c->Y1h, (const u16*)c->Y1l, st.colw, c->epi, np, c->Mp, chain0,
                      nrt, c->wpart);
 }
