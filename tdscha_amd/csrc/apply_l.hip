@@ -9,6 +9,7 @@
 // pairs p in row-major upper-triangle order (row r <= col c), w_a = w[c],
 // w_b = w[r] (DL:2339-2346).
 #include <cmath>
+#include <map>
 #include <vector>
 
 #include "common.hpp"
@@ -260,6 +261,41 @@ __global__ void perm_average(const double* __restrict__ Ccart, const int* __rest
   Cavg[chain * nn + e] = s / n_cells;
 }
 
+// The same average when the permutations form a group H: for dof J in the
+// orbit of k = rep_dof[jrep[J]] with inv_perms[u](J) = k (u = ju[J]),
+//   Cavg[J, K] = S[k][inv_perms[u](K)],  S[k][K'] = mean_v C[ip_v(k), ip_v(K')]
+// (substitute t = v o u in the sum over t), so only the n_rep representative
+// rows are averaged.
+__global__ void perm_rep_sum(const double* __restrict__ Ccart, const int* __restrict__ inv_perms,
+                             const int* __restrict__ rep_dof, int n_rep, int n_cart, int n_cells,
+                             double* __restrict__ S) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n_rep * n_cart) return;
+  const int r = (int)(e / n_cart), K = (int)(e % n_cart);
+  const int k = rep_dof[r];
+  const long nn = (long)n_cart * n_cart;
+  const double* C = Ccart + chain * nn;
+  double s = 0.0;
+  for (int t = 0; t < n_cells; ++t) {
+    const int* ip = inv_perms + (long)t * n_cart;
+    s += C[(long)ip[k] * n_cart + ip[K]];
+  }
+  S[(long)chain * n_rep * n_cart + e] = s / n_cells;
+}
+
+__global__ void perm_expand(const double* __restrict__ S, const int* __restrict__ inv_perms,
+                            const int* __restrict__ jrep, const int* __restrict__ ju, int n_rep,
+                            int n_cart, double* __restrict__ Cavg) {
+  const int chain = blockIdx.y;
+  const long nn = (long)n_cart * n_cart;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  const int J = (int)(e / n_cart), K = (int)(e % n_cart);
+  const int Kp = inv_perms[(long)ju[J] * n_cart + K];
+  Cavg[chain * nn + e] = S[((long)chain * n_rep + jrep[J]) * n_cart + Kp];
+}
+
 // f <- P_T f  (P_T = mean of the mode-space translation operators); tprojT
 // holds P_T transposed so that consecutive threads read consecutive entries
 // out = P_T f: a block of 32 output modes, the K = n sum split over 8 thread
@@ -294,8 +330,17 @@ int project_translations(Ctx* c, int nc, hipStream_t s) {
   gemm(nc, nc_, nc_, n, c->mode_tmp, n, (long)nc_ * n, 0, c->pols, n, 0, 1, c->cart_tmp, nc_,
        (long)nc_ * nc_, s);
   const long nn = (long)nc_ * nc_;
-  hipLaunchKernelGGL(perm_average, dim3((unsigned)((nn + 255) / 256), nc), dim3(256), 0, s,
-                     c->cart_tmp, c->inv_perms, nc_, c->n_cells, c->cart_tmp2);
+  if (c->n_rep > 0) {
+    const long nr = (long)c->n_rep * nc_;
+    hipLaunchKernelGGL(perm_rep_sum, dim3((unsigned)((nr + 255) / 256), nc), dim3(256), 0, s,
+                       c->cart_tmp, c->inv_perms, c->rep_dof, c->n_rep, nc_, c->n_cells,
+                       c->rep_sum);
+    hipLaunchKernelGGL(perm_expand, dim3((unsigned)((nn + 255) / 256), nc), dim3(256), 0, s,
+                       c->rep_sum, c->inv_perms, c->jrep, c->ju, c->n_rep, nc_, c->cart_tmp2);
+  } else {
+    hipLaunchKernelGGL(perm_average, dim3((unsigned)((nn + 255) / 256), nc), dim3(256), 0, s,
+                       c->cart_tmp, c->inv_perms, nc_, c->n_cells, c->cart_tmp2);
+  }
   // mode_tmp = pols^T * Cavg         (n x n_cart)
   gemm(nc, n, nc_, nc_, c->pols, n, 0, 1, c->cart_tmp2, nc_, nn, 0, c->mode_tmp, nc_,
        (long)n * nc_, s);
@@ -419,6 +464,53 @@ int tdscha_set_translations(tdscha_ctx* ctx, int n_cart, const double* pols, int
     hipFree(c->pols); hipFree(c->inv_perms); hipFree(c->tproj); hipFree(c->cart_tmp);
     hipFree(c->cart_tmp2); hipFree(c->mode_tmp); hipFree(c->f_proj);
     c->pols = nullptr;
+  }
+  if (c->rep_dof) {
+    hipFree(c->rep_dof); hipFree(c->jrep); hipFree(c->ju); hipFree(c->rep_sum);
+    c->rep_dof = c->jrep = c->ju = nullptr;
+    c->rep_sum = nullptr;
+  }
+  c->n_rep = 0;
+  // orbit tables when the permutations form a group (closed under
+  // composition); otherwise the direct average is used
+  {
+    std::map<std::vector<int>, int> index;
+    for (int t = 0; t < n_cells; ++t)
+      index[std::vector<int>(inv_perms + (long)t * n_cart, inv_perms + (long)(t + 1) * n_cart)] = t;
+    bool group = (int)index.size() == n_cells;
+    std::vector<int> comp(n_cart);
+    for (int t1 = 0; group && t1 < n_cells; ++t1)
+      for (int t2 = 0; group && t2 < n_cells; ++t2) {
+        for (int i = 0; i < n_cart; ++i)
+          comp[i] = inv_perms[(long)t1 * n_cart + inv_perms[(long)t2 * n_cart + i]];
+        group = index.count(comp) > 0;
+      }
+    if (group) {
+      std::vector<int> jr(n_cart), jt(n_cart), reps, rep_of(n_cart, -1);
+      for (int J = 0; J < n_cart; ++J) {
+        int best = n_cart, bt = 0;
+        for (int t = 0; t < n_cells; ++t) {
+          const int k = inv_perms[(long)t * n_cart + J];
+          if (k < best) { best = k; bt = t; }
+        }
+        if (rep_of[best] < 0) {
+          rep_of[best] = (int)reps.size();
+          reps.push_back(best);
+        }
+        jr[J] = rep_of[best];
+        jt[J] = bt;
+      }
+      c->n_rep = (int)reps.size();
+      TDS_CHECK_HIP(hipMalloc((void**)&c->rep_dof, sizeof(int) * c->n_rep));
+      TDS_CHECK_HIP(hipMalloc((void**)&c->jrep, sizeof(int) * n_cart));
+      TDS_CHECK_HIP(hipMalloc((void**)&c->ju, sizeof(int) * n_cart));
+      TDS_CHECK_HIP(hipMalloc((void**)&c->rep_sum,
+                              sizeof(double) * kMaxChains * (long)c->n_rep * n_cart));
+      TDS_CHECK_HIP(hipMemcpy(c->rep_dof, reps.data(), sizeof(int) * c->n_rep,
+                              hipMemcpyHostToDevice));
+      TDS_CHECK_HIP(hipMemcpy(c->jrep, jr.data(), sizeof(int) * n_cart, hipMemcpyHostToDevice));
+      TDS_CHECK_HIP(hipMemcpy(c->ju, jt.data(), sizeof(int) * n_cart, hipMemcpyHostToDevice));
+    }
   }
   c->n_cart = n_cart;
   TDS_CHECK_HIP(hipMalloc((void**)&c->pols, sizeof(double) * (long)n_cart * n));

@@ -128,6 +128,13 @@ struct Ctx {
   int n_cart = 0;
   double* pols = nullptr;      // [n_cart, n]
   int* inv_perms = nullptr;    // [n_cells, n_cart]
+  // translation group orbits (when the permutations form a group): dof J
+  // lies in the orbit of rep_dof[jrep[J]], reached by inv_perms[ju[J]]
+  int n_rep = 0;
+  int* rep_dof = nullptr;      // [n_rep]
+  int* jrep = nullptr;         // [n_cart]
+  int* ju = nullptr;           // [n_cart]
+  double* rep_sum = nullptr;   // [kMaxChains, n_rep, n_cart]
   double* tproj = nullptr;     // [n, n]
   double* cart_tmp = nullptr;  // [kMaxChains, n_cart, n_cart] x2
   double* cart_tmp2 = nullptr;
