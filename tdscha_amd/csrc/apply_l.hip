@@ -277,6 +277,7 @@ __global__ void perm_rep_sum(const double* __restrict__ Ccart, const int* __rest
   const long nn = (long)n_cart * n_cart;
   const double* C = Ccart + chain * nn;
   double s = 0.0;
+#pragma unroll 8
   for (int t = 0; t < n_cells; ++t) {
     const int* ip = inv_perms + (long)t * n_cart;
     s += C[(long)ip[k] * n_cart + ip[K]];
@@ -308,8 +309,10 @@ __global__ __launch_bounds__(256) void project_f(const double* __restrict__ tpro
   const int m = blockIdx.x * 32 + tx;
   const double* f = f_all + (long)chain * n;
   double s = 0.0;
-  if (m < n)
+  if (m < n) {
+#pragma unroll 8
     for (int k = ty; k < n; k += 8) s += tprojT[(long)k * n + m] * f[k];
+  }
   __shared__ double red[8][32];
   red[ty][tx] = s;
   __syncthreads();

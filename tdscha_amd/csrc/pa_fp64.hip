@@ -294,8 +294,17 @@ __global__ void reduce_ksplit(const double* __restrict__ zpart_all,
   const long nn = (long)np * np;
   const double* zp = zpart_all + (long)chain * ksplit * nn;
   if (e < nn) {
+    // eight loads in flight, summed in the same order as one at a time
     double s = 0.0;
-    for (int k = 0; k < ksplit; ++k) s += zp[(long)k * nn + e];
+    int k = 0;
+    for (; k + 8 <= ksplit; k += 8) {
+      double v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = zp[(long)(k + j) * nn + e];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += v[j];
+    }
+    for (; k < ksplit; ++k) s += zp[(long)k * nn + e];
     zsum_all[(long)chain * nn + e] = s;
   }
   if (e < np) {

@@ -878,8 +878,17 @@ __global__ void reduce_ksplit_s32(const float* __restrict__ zpart_all,
   const long nn = (long)np * np;
   const float* zp = zpart_all + (long)chain * ksplit * nn;
   if (e < nn) {
+    // eight loads in flight, summed in the same order as one at a time
     double s = 0.0;
-    for (int k = 0; k < ksplit; ++k) s += (double)zp[(long)k * nn + e];
+    int k = 0;
+    for (; k + 8 <= ksplit; k += 8) {
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = zp[(long)(k + j) * nn + e];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += (double)v[j];
+    }
+    for (; k < ksplit; ++k) s += (double)zp[(long)k * nn + e];
     const int mu = (int)(e / np), nu = (int)(e % np);
     zsum_all[(long)chain * nn + e] = s / (su[mu] * sb_all[(long)chain * np + nu]);
   }
