@@ -1,0 +1,131 @@
+"""Ensemble preprocessing into the mode-space data the Lanczos consumes.
+
+Restates, on plain arrays, what ``DynamicalLanczos.Lanczos.__init__`` does with
+a sscha ``Ensemble`` (Modules/DynamicalLanczos.py:322-452):
+
+  * drop the rigid translations from the supercell modes (DL:340-352; the
+    reference calls CellConstructor's ``get_translations`` on the
+    mass-scaled eigenvectors) and keep an optional ``select_modes`` subset
+    (DL:355-366);
+  * residual forces f - f_sscha (DL:394-395), minus their average over the
+    ensemble and the supercell images per unit-cell atom, symmetrised by the
+    point group and tiled over the supercell (DL:398-419, 441);
+  * mass scaling u~ = sqrt(m) u, f~ = f / sqrt(m) (DL:444-445) and the mode
+    projections X = u~ pols, Y = f~ pols (DL:451-452).
+
+Units are Rydberg atomic units: displacements in Bohr, forces in Ry/Bohr,
+masses in Rydberg mass units (2 m_e); ``angstrom=True`` takes displacements
+in Angstrom and forces in Ry/Angstrom.  Supercell atoms are ordered cell by
+cell (atom k of cell l at index l * nat_uc + k), the order of the unit cell
+repeated.  The result has the attributes the ``Lanczos`` constructor reads
+(T, w, pols, m, N, rho, X, Y).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+BOHR_ANGSTROM = 0.52917720859
+
+
+def translation_mask(pols, m, threshold=0.5):
+    """Modes that are rigid translations: total overlap with the three
+    mass-weighted uniform translations above ``threshold`` (pols are the
+    mass-scaled eigenvectors, columns; m the mass of every degree of
+    freedom)."""
+    pols = np.asarray(pols, float)
+    m = np.asarray(m, float)
+    t = np.zeros((len(m), 3))
+    for c in range(3):
+        t[c::3, c] = np.sqrt(m[c::3])
+    t /= np.linalg.norm(t, axis=0)
+    overlap = np.sum((t.T @ pols) ** 2, axis=0)
+    return overlap > threshold
+
+
+def symmetrize_unit_cell_vector(v, rotations, atom_maps):
+    """Point-group average of a per-atom vector v (nat_uc, 3): operation j
+    moves atom k to atom_maps[j][k] and rotates by rotations[j]
+    (the role of QE_Symmetry.SymmetrizeVector at DL:408-410)."""
+    v = np.asarray(v, float)
+    out = np.zeros_like(v)
+    for R, irt in zip(rotations, atom_maps):
+        rv = v @ np.asarray(R, float).T
+        out[np.asarray(irt)] += rv
+    return out / len(rotations)
+
+
+class PreparedEnsemble:
+    """Mode-space ensemble (the attributes the Lanczos constructor reads)."""
+
+    def __init__(self, T, w, pols, m, rho, X, Y, u_tilde, f_tilde, mode_mask):
+        self.T = float(T)
+        self.w = w
+        self.pols = pols
+        self.m = m
+        self.rho = rho
+        self.N = len(rho)
+        self.N_eff = float(np.sum(rho))
+        self.X = X
+        self.Y = Y
+        self.u_tilde = u_tilde
+        self.f_tilde = f_tilde
+        self.mode_mask = mode_mask
+
+
+def prepare_ensemble(u_disps, forces, sscha_forces, masses, w, pols, T, rho=None, n_cells=1,
+                     rotations=None, atom_maps=None, select_modes=None, ignore_small_w=False,
+                     small_w=1e-8, angstrom=False):
+    """Mode-space ensemble from raw configurations.
+
+    u_disps, forces, sscha_forces: (N, nat, 3) or (N, 3 nat) displacements
+    from the average positions, forces and SSCHA (harmonic) forces of the
+    supercell; masses: (nat,) per atom; w, pols: all 3 nat supercell
+    frequencies (Ry) and mass-scaled eigenvectors (columns); rho: weights
+    (default 1); n_cells: unit cells in the supercell; rotations,
+    atom_maps: the unit cell's point group for the average-force
+    symmetrisation (skipped if None)."""
+    u = np.asarray(u_disps, float)
+    N = u.shape[0]
+    u = u.reshape(N, -1).copy()
+    f = np.asarray(forces, float).reshape(N, -1) - np.asarray(sscha_forces, float).reshape(N, -1)
+    if angstrom:                           # DL:389-395, 421
+        u /= BOHR_ANGSTROM
+        f *= BOHR_ANGSTROM
+    n3 = u.shape[1]
+    masses = np.asarray(masses, float)
+    if 3 * len(masses) != n3:
+        raise ValueError("masses must give one entry per atom of the supercell")
+    m = np.repeat(masses, 3)               # DL:336
+    w = np.asarray(w, float)
+    pols = np.asarray(pols, float)
+    if pols.shape != (n3, len(w)):
+        raise ValueError("pols must be (3 nat, n_modes) with one column per frequency")
+    if ignore_small_w:                     # DL:347-350
+        trans = np.abs(w) < small_w
+    else:
+        trans = translation_mask(pols, m)
+    good = ~trans
+    if select_modes is not None:           # DL:355-366
+        select_modes = np.asarray(select_modes, bool)
+        if len(select_modes) != len(w):
+            raise ValueError("'select_modes' should be an array of the same length of the "
+                             "number of modes.")
+        good &= select_modes
+    rho = np.ones(N) if rho is None else np.asarray(rho, float).copy()
+    if len(rho) != N:
+        raise ValueError("rho must have one weight per configuration")
+    if n3 % (3 * n_cells):
+        raise ValueError("the supercell atoms are not a whole number of unit cells")
+    nat_uc = n3 // (3 * n_cells)
+    # average residual force per unit-cell atom (DL:398-419)
+    fm = (rho @ f) / rho.sum()
+    fm = fm.reshape(n_cells, nat_uc, 3).mean(axis=0)
+    if rotations is not None:
+        fm = symmetrize_unit_cell_vector(fm, rotations, atom_maps)
+    f -= np.tile(fm.ravel(), n_cells)[None]       # DL:441
+    u *= np.sqrt(m)[None]                          # DL:444-445
+    f /= np.sqrt(m)[None]
+    pg = np.ascontiguousarray(pols[:, good])
+    X = np.ascontiguousarray(u @ pg)               # DL:451-452
+    Y = np.ascontiguousarray(f @ pg)
+    return PreparedEnsemble(T, w[good].copy(), pg, m, rho, X, Y, u, f, good)
