@@ -77,6 +77,31 @@ def test_apply_L_gamma_only_projection(snte):
     lanc.close()
 
 
+def test_projection_with_a_permutation_set_that_is_not_a_group(snte):
+    """Three of the eight translations are not closed under composition: the
+    direct average runs instead of the orbit sums, and matches the oracle fed
+    the same three permutations."""
+    import copy
+    from oracle import lanczos_oracle as O
+    from tdscha_amd.lanczos import Lanczos
+    ms = copy.copy(snte["syms_gamma"])
+    ms.trans_cart_perms = [np.asarray(p) for p in snte["syms_gamma"].trans_cart_perms[:3]]
+    lanc = Lanczos()
+    lanc.load_status(os.path.join(GOLDEN, "snte_step10.npz"))
+    lanc.use_wigner = True
+    lanc.gamma_only = True
+    lanc.init(use_symmetries=True, symmetries=ms)
+    lanc.reset()
+    orc = O.OracleSystem(snte["X"], snte["Y"], snte["w"], snte["rho"], float(snte["T"]),
+                         ms.dense(), use_wigner=True, pols=snte["pols"],
+                         trans_projector=ms.trans_projector, inv_perms=ms.trans_cart_perms)
+    v = np.random.default_rng(5).normal(size=len(snte["psi"]))
+    err = rel(lanc.apply_full_L(v.copy()), orc.apply_full_L(v))
+    print("non-group projection rel err %.2e" % err)
+    assert err < 1e-12
+    lanc.close()
+
+
 def test_run_FT_reproduces_reference_golden_run(snte):
     """Biconjugate standard-representation run of test_julia_lf.py:24-45."""
     lanc = _product_from_fixture(snte, use_wigner=False)
