@@ -173,8 +173,8 @@ void tdscha_qctx_destroy(tdscha_qctx* ctx);
  * n_bands complex block D[((j * n_q + p) * n_bands + nu) * n_bands + mu]
  * (interleaved re, im).  Expands the context's ensemble on the device into
  * its n_sym * n_configs images (image j * n_configs + i), normalised by
- * n_sym * sum(rho); call once, before any run.  Rows of q_perm that are not
- * permutations are rejected.
+ * n_sym * sum(rho); once per context (a second call is rejected), before any
+ * run.  Rows of q_perm that are not permutations are rejected.
  */
 int tdscha_qctx_symmetrize(tdscha_qctx* ctx, int n_sym, const int* q_perm, const double* D);
 

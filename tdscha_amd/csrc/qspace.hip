@@ -42,6 +42,7 @@ constexpr int kQMaxQ = 512;
 
 struct QCtx {
   int nq = 0, nb = 0, N = 0;
+  int n_sym = 1;            // point-group images expanded so far (1: none)
   double T = 0.0, neff = 0.0;
   double2* X = nullptr;     // [nq][nb][N]
   double2* Y = nullptr;
@@ -933,6 +934,7 @@ int tdscha_qctx_symmetrize(tdscha_qctx* ctx, int n_sym, const int* q_perm, const
   QCtx* q = reinterpret_cast<QCtx*>(ctx);
   const int nq = q->nq, nb = q->nb, N = q->N;
   TDS_REQUIRE(n_sym >= 1 && n_sym <= 65535, 2, "tdscha_qctx_symmetrize: n_sym must be in [1, 65535]");
+  TDS_REQUIRE(q->n_sym == 1, 2, "tdscha_qctx_symmetrize: the ensemble is already expanded");
   TDS_REQUIRE((long)N * n_sym <= 0x7fffffffL, 2,
               "tdscha_qctx_symmetrize: n_sym * n_configs exceeds 2^31 - 1");
   for (int j = 0; j < n_sym; ++j) {
@@ -1001,6 +1003,7 @@ int tdscha_qctx_symmetrize(tdscha_qctx* ctx, int n_sym, const int* q_perm, const
   q->rho = rho2;
   q->N = N2;
   q->neff *= n_sym;
+  q->n_sym = n_sym;
   return 0;
 }
 

@@ -187,6 +187,11 @@ def test_symmetrized_lanczos_and_mirror_match_oracle():
     perm, D = sym
     bad = perm[:2].copy()
     bad[1, 0] = bad[1, 1]
+    with pytest.raises(RuntimeError, match="already expanded"):
+        lanc.engine.symmetrize(perm, D)
+    from tdscha_amd.qlanczos import QSpaceEngine
+    fresh = QSpaceEngine(lanc.X_q, lanc.Y_q, lanc.w_q, lanc.valid_modes_q, lanc.rho, lanc.T)
     with pytest.raises(RuntimeError, match="not a permutation"):
-        lanc.engine.symmetrize(bad, D[:2])
+        fresh.symmetrize(bad, D[:2])
+    fresh.close()
     lanc.close()
