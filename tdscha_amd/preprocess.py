@@ -18,7 +18,9 @@ masses in Rydberg mass units (2 m_e); ``angstrom=True`` takes displacements
 in Angstrom and forces in Ry/Angstrom.  Supercell atoms are ordered cell by
 cell (atom k of cell l at index l * nat_uc + k), the order of the unit cell
 repeated.  The result has the attributes the ``Lanczos`` constructor reads
-(T, w, pols, m, N, rho, X, Y).
+(T, w, pols, m, N, rho, X, Y) and, when the supercell structure is given, a
+``symmetries`` method that detects the space group (symmetry.find_symmetries)
+for ``Lanczos.init(use_symmetries=True)``.
 """
 from __future__ import annotations
 
@@ -57,7 +59,7 @@ def symmetrize_unit_cell_vector(v, rotations, atom_maps):
 class PreparedEnsemble:
     """Mode-space ensemble (the attributes the Lanczos constructor reads)."""
 
-    def __init__(self, T, w, pols, m, rho, X, Y, u_tilde, f_tilde, mode_mask):
+    def __init__(self, T, w, pols, m, rho, X, Y, u_tilde, f_tilde, mode_mask, structure=None):
         self.T = float(T)
         self.w = w
         self.pols = pols
@@ -70,11 +72,31 @@ class PreparedEnsemble:
         self.u_tilde = u_tilde
         self.f_tilde = f_tilde
         self.mode_mask = mode_mask
+        self.structure = structure
+
+    def symmetries(self, use_symmetries=True, gamma_only=True):
+        """Mode-space symmetries of the supercell (DL:694-871): detected space
+        group, point group + translation projector if ``gamma_only``."""
+        from . import symmetry as SY
+        n = len(self.w)
+        if not use_symmetries:
+            syms = [np.ones((1, 1, 1)) for _ in range(n)]
+            blocks = [np.array([i], dtype=np.int32) for i in range(n)]
+            return SY.ModeSymmetries(syms, blocks, n)
+        if self.structure is None:
+            raise ValueError("symmetries need the supercell structure (lattice, positions, "
+                             "species) passed to prepare_ensemble")
+        lattice, positions, species = self.structure
+        ms = SY.detect_mode_symmetries(lattice, positions, species, self.pols, self.w,
+                                       gamma_only=gamma_only)
+        if ms.leakage > 1e-8:
+            raise RuntimeError("symmetry not block diagonal (leakage %.2e)" % ms.leakage)
+        return ms
 
 
 def prepare_ensemble(u_disps, forces, sscha_forces, masses, w, pols, T, rho=None, n_cells=1,
                      rotations=None, atom_maps=None, select_modes=None, ignore_small_w=False,
-                     small_w=1e-8, angstrom=False):
+                     small_w=1e-8, angstrom=False, structure=None):
     """Mode-space ensemble from raw configurations.
 
     u_disps, forces, sscha_forces: (N, nat, 3) or (N, 3 nat) displacements
@@ -83,7 +105,9 @@ def prepare_ensemble(u_disps, forces, sscha_forces, masses, w, pols, T, rho=None
     frequencies (Ry) and mass-scaled eigenvectors (columns); rho: weights
     (default 1); n_cells: unit cells in the supercell; rotations,
     atom_maps: the unit cell's point group for the average-force
-    symmetrisation (skipped if None)."""
+    symmetrisation (skipped if None); structure: (lattice, positions,
+    species) of the supercell, Bohr (Angstrom with ``angstrom``), for the
+    symmetry detection."""
     u = np.asarray(u_disps, float)
     N = u.shape[0]
     u = u.reshape(N, -1).copy()
@@ -128,4 +152,12 @@ def prepare_ensemble(u_disps, forces, sscha_forces, masses, w, pols, T, rho=None
     pg = np.ascontiguousarray(pols[:, good])
     X = np.ascontiguousarray(u @ pg)               # DL:451-452
     Y = np.ascontiguousarray(f @ pg)
-    return PreparedEnsemble(T, w[good].copy(), pg, m, rho, X, Y, u, f, good)
+    if structure is not None:
+        lat, pos, spec = structure
+        lat = np.asarray(lat, float)
+        pos = np.asarray(pos, float)
+        if angstrom:
+            lat = lat / BOHR_ANGSTROM
+            pos = pos / BOHR_ANGSTROM
+        structure = (lat, pos, np.asarray(spec))
+    return PreparedEnsemble(T, w[good].copy(), pg, m, rho, X, Y, u, f, good, structure)

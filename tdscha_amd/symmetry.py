@@ -123,31 +123,89 @@ class ModeSymmetries:
         return S
 
 
-def build_mode_symmetries(positions, lattice, center, pols, w,
-                          translations, gamma_only=False, rtol=1e-6):
-    """Mode-space symmetries of a rock-salt supercell.
+def find_symmetries(lattice, positions, species, tol=1e-5):
+    """Space-group operations of a periodic structure: the role spglib plays in
+    the reference (``spglib.get_symmetry``, used by the symmetry setup at
+    DL:694-871 and QL:1498-1515).
 
-    ``translations``: list of Cartesian lattice translations of the primitive
-    cell inside the supercell (including 0).  If ``gamma_only`` the images
-    use only the 48 point-group operations (t = 0) and the translations are
-    returned as a projector (DL:747-791); otherwise all 48 x N_cell
-    operations are images.
-    """
-    rots = cubic_rotations()
+    ``lattice`` (3, 3) rows = cell vectors, ``positions`` (nat, 3) Cartesian,
+    ``species`` (nat,) labels.  The rotations are the integer matrices M of
+    the lattice basis (entries in {-1, 0, 1}, enough for a reduced cell)
+    whose Cartesian form R = (A^-1 M A)^T is orthogonal; for each, every
+    translation t (modulo the cell) that sends each atom onto an atom of the
+    same species.  Returns a list of (R, t, irt), r -> R r + t, atom i ->
+    irt[i], identity first.  For a supercell the pure lattice translations
+    inside it appear as (I, t)."""
+    A = np.asarray(lattice, float)
+    Ai = np.linalg.inv(A)
+    pos = np.asarray(positions, float)
+    spec = np.asarray(species)
+    frac = pos @ Ai
+    scale = np.linalg.norm(A, axis=1).max()
+    rots = []
+    for e in itertools.product((-1, 0, 1), repeat=9):
+        M = np.array(e, float).reshape(3, 3)
+        if abs(abs(np.linalg.det(M)) - 1.0) > 1e-8:
+            continue
+        R = (Ai @ M @ A).T
+        if np.allclose(R @ R.T, np.eye(3), atol=1e-8):
+            rots.append(R)
+    rots.sort(key=lambda R: (not np.allclose(R, np.eye(3)),))
+    ops = []
+    for R in rots:
+        newf = (pos @ R.T) @ Ai                          # fractional images, before t
+        for k in np.flatnonzero(spec == spec[0]):
+            tf = frac[k] - newf[0]                         # candidate: atom 0 -> atom k
+            img = newf + tf
+            d = img[:, None, :] - frac[None, :, :]
+            d -= np.round(d)
+            dist = np.linalg.norm(d @ A, axis=2)
+            irt = np.argmin(dist, axis=1)
+            if (dist[np.arange(len(pos)), irt] > tol * scale).any():
+                continue
+            if (spec[irt] != spec).any() or len(set(irt.tolist())) != len(irt):
+                continue
+            tf = tf - np.floor(tf + 1e-9)
+            ops.append((R, tf @ A, irt))
+    return ops
+
+
+def split_point_group(ops, lattice, tol=1e-8):
+    """Pure translations T = {t : (I, t)} and one representative (R, t) per
+    distinct rotation, t the shortest of its coset t + T (0 where the group
+    is symmorphic about the origin)."""
+    A = np.asarray(lattice, float)
+    Ai = np.linalg.inv(A)
+    trans = [t for R, t, _ in ops if np.allclose(R, np.eye(3), atol=tol)]
+    reps = []
+    for R, t, irt in ops:
+        if any(np.allclose(R, Rr, atol=tol) for Rr, _, _ in reps):
+            continue
+        best = None
+        for R2, t2, irt2 in ops:
+            if not np.allclose(R2, R, atol=tol):
+                continue
+            f = t2 @ Ai
+            f = f - np.round(f)
+            n2 = np.linalg.norm(f @ A)
+            if best is None or n2 < best[0] - tol:
+                best = (n2, t2, irt2)
+        reps.append((R, best[1], best[2]))
+    return trans, reps
+
+
+def build_mode_symmetries_from_ops(positions, lattice, pols, w, ops, translations=None,
+                                   gamma_only=False, rtol=1e-6):
+    """Mode-space symmetries from explicit operations (R, t) (r -> R r + t):
+    ``ops`` are the images; with ``gamma_only`` the ``translations`` are
+    returned as the projector (DL:747-791) instead of images."""
     blocks = degenerate_blocks(w, rtol)
     n = pols.shape[1]
-    ops = []
-    if gamma_only:
-        for R in rots:
-            ops.append((R, np.zeros(3)))
-    else:
-        for R in rots:
-            for t in translations:
-                ops.append((R, np.asarray(t, float)))
+    zero = np.zeros(3)
     mats = [[] for _ in blocks]
     leak = 0.0
     for R, t in ops:
-        irt = atom_permutation(positions, lattice, R, center, t)
+        irt = atom_permutation(positions, lattice, R, zero, np.asarray(t, float))
         S = pols.T @ cartesian_operator(irt, R) @ pols
         mask = np.zeros((n, n), dtype=bool)
         for bi, b in enumerate(blocks):
@@ -160,7 +218,7 @@ def build_mode_symmetries(positions, lattice, center, pols, w,
         tops, tperms = [], []
         I3 = np.eye(3)
         for t in translations:
-            irt = atom_permutation(positions, lattice, I3, center, np.asarray(t, float))
+            irt = atom_permutation(positions, lattice, I3, zero, np.asarray(t, float))
             P = cartesian_operator(irt, I3)
             nat = len(irt)
             inv_perm = np.zeros(3 * nat, dtype=np.int64)
@@ -170,3 +228,40 @@ def build_mode_symmetries(positions, lattice, center, pols, w,
             tops.append(pols.T @ P @ pols)
         tp = np.mean(tops, axis=0)
     return ModeSymmetries(symmetries, blocks, n, tp, tops, tperms, leak)
+
+
+def detect_mode_symmetries(lattice, positions, species, pols, w, gamma_only=False, tol=1e-5,
+                           rtol=1e-6):
+    """find_symmetries + build_mode_symmetries_from_ops for a supercell:
+    every space-group operation as an image, or (``gamma_only``) the point
+    group representatives as images and the pure translations as the
+    projector."""
+    ops = find_symmetries(lattice, positions, species, tol)
+    if gamma_only:
+        trans, reps = split_point_group(ops, lattice)
+        return build_mode_symmetries_from_ops(positions, lattice, pols, w,
+                                              [(R, t) for R, t, _ in reps], trans,
+                                              gamma_only=True, rtol=rtol)
+    return build_mode_symmetries_from_ops(positions, lattice, pols, w,
+                                          [(R, t) for R, t, _ in ops], rtol=rtol)
+
+
+def build_mode_symmetries(positions, lattice, center, pols, w,
+                          translations, gamma_only=False, rtol=1e-6):
+    """Mode-space symmetries of a rock-salt supercell: the 48 rotations of
+    O_h about ``center`` (r -> R (r - c) + c).
+
+    ``translations``: list of Cartesian lattice translations of the primitive
+    cell inside the supercell (including 0).  If ``gamma_only`` the images
+    use only the 48 point-group operations and the translations are
+    returned as a projector (DL:747-791); otherwise all 48 x N_cell
+    operations are images.
+    """
+    c = np.asarray(center, float)
+    if gamma_only:
+        ops = [(R, c - R @ c) for R in cubic_rotations()]
+    else:
+        ops = [(R, c - R @ c + np.asarray(t, float)) for R in cubic_rotations()
+               for t in translations]
+    return build_mode_symmetries_from_ops(positions, lattice, pols, w, ops, translations,
+                                          gamma_only=gamma_only, rtol=rtol)

@@ -196,3 +196,34 @@ def test_continued_fraction_kernel_matches_oracle():
     got = continued_fraction(a, b[:-1], None, w, True, True, 3, 0.0)
     ref = O.continued_fraction(a, b[:-1], [], w, True, True, 3, 0.0)
     assert rel(got, ref) < 1e-12
+
+
+def test_lanczos_from_a_prepared_ensemble_with_detected_symmetries():
+    """Raw arrays -> prepare_ensemble (with the supercell structure) -> Lanczos
+    with detected point group + translation projector: the same L as the
+    rock-salt ensemble with the builder's symmetries."""
+    from tdscha_amd import preprocess as PP
+    from tdscha_amd import symmetry as SY
+    from tdscha_amd.lanczos import Lanczos
+    from tdscha_amd.rocksalt import Ensemble, RockSalt
+    model = RockSalt(L=2)
+    ens = Ensemble(model, 300.0, 40, seed=2)
+    u = ens.u_disps
+    f = model.forces(u.reshape(40, model.nat, 3)).reshape(40, -1)
+    fs = -(u @ model.force_constants().T)
+    w, pols = model.harmonic_modes()
+    rots = SY.cubic_rotations()
+    pe = PP.prepare_ensemble(u, f, fs, model.m_atoms, w, pols, ens.T, n_cells=model.n_cells,
+                             rotations=rots, atom_maps=[np.arange(2)] * 48,
+                             structure=(model.lattice, model.positions, model.species))
+    out = []
+    for e in (pe, ens):
+        lanc = Lanczos(e)
+        lanc.gamma_only = True
+        lanc.init(use_symmetries=True)
+        v = np.random.default_rng(9).normal(size=len(lanc.psi))
+        out.append(lanc.apply_full_L(v.copy()))
+        lanc.close()
+    err = rel(out[0], out[1])
+    print("prepared + detected vs generator + builder: rel %.2e" % err)
+    assert err < 1e-11

@@ -74,3 +74,35 @@ def test_mode_projection_reconstructs_the_vibrational_part():
     pe2 = PP.prepare_ensemble(u + shift, f, fs, model.m_atoms, w, pols, ens.T,
                               n_cells=model.n_cells)
     np.testing.assert_allclose(pe2.X, pe.X, atol=1e-12 * np.abs(pe.X).max())
+
+
+def test_prepared_ensemble_detects_its_symmetries():
+    """With the supercell structure the prepared ensemble finds its space group:
+    the gamma-only point group + projector equal the rock-salt builder's."""
+    from oracle import lanczos_oracle as LO
+    model, ens, u, f, fs, w, pols, rots, maps = _raw(N=10)
+    pe = PP.prepare_ensemble(u, f, fs, model.m_atoms, w, pols, ens.T, n_cells=model.n_cells,
+                             rotations=rots, atom_maps=maps,
+                             structure=(model.lattice, model.positions, model.species))
+    det = pe.symmetries(True, gamma_only=True)
+    ref = ens.symmetries(True, gamma_only=True)
+    assert det.n_syms == 48 and len(det.trans_cart_perms) == model.n_cells
+    np.testing.assert_allclose(det.trans_projector, ref.trans_projector, atol=1e-13)
+    o1 = LO.OracleSystem(pe.X, pe.Y, pe.w, pe.rho, pe.T, det.dense(), use_wigner=True,
+                         pols=pe.pols, trans_projector=det.trans_projector,
+                         inv_perms=det.trans_cart_perms)
+    o2 = LO.OracleSystem(ens.X, ens.Y, ens.w, ens.rho, ens.T, ref.dense(), use_wigner=True,
+                         pols=ens.pols, trans_projector=ref.trans_projector,
+                         inv_perms=ref.trans_cart_perms)
+    v = np.random.default_rng(3).normal(size=LO.len_psi(len(pe.w)))
+    a, b = o1.apply_full_L(v), o2.apply_full_L(v)
+    assert np.linalg.norm(a - b) / np.linalg.norm(b) < 1e-11
+    # in Angstrom the structure is converted like the displacements
+    pa = PP.prepare_ensemble(u * PP.BOHR_ANGSTROM, f / PP.BOHR_ANGSTROM, fs / PP.BOHR_ANGSTROM,
+                             model.m_atoms, w, pols, ens.T, n_cells=model.n_cells, angstrom=True,
+                             structure=(model.lattice * PP.BOHR_ANGSTROM,
+                                        model.positions * PP.BOHR_ANGSTROM, model.species))
+    assert pa.symmetries(True, gamma_only=True).n_syms == 48
+    with pytest.raises(ValueError):
+        PP.prepare_ensemble(u, f, fs, model.m_atoms, w, pols, ens.T,
+                            n_cells=model.n_cells).symmetries(True)
