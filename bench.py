@@ -154,16 +154,23 @@ def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True):
     if per_chain:
         tiles1, _ = eng.autotune(iqs[:1], pms[:1], reps=2)     # tiles for one chain at a time
         t = time.perf_counter()
-        worst = 0.0
+        worst, worst50 = 0.0, 0.0
         for i in range(nq):
             (a, b, c), = eng.lanczos([iqs[i]], [pms[i]], [psis[i]], int(caps[i]), tol=tol,
                                      use_graph=False)[0]
             k = min(len(a), len(chains[i][0]))
-            worst = max(worst, float(np.max(np.abs(a[:k] - chains[i][0][:k]) /
-                                            np.max(np.abs(chains[i][0][:k])))))
+            d = np.abs(a[:k] - chains[i][0][:k]) / np.max(np.abs(chains[i][0][:k]))
+            worst = max(worst, float(np.max(d)))
+            worst50 = max(worst50, float(np.max(d[:50])))
         wall1 = time.perf_counter() - t
+        # The per-chain run sums in the order of its own tiles: the first 50
+        # coefficients agree to rounding; late coefficients of a few chains
+        # depend on that order at the 1e-2 level past step ~200 (rounding
+        # differences grow along long recurrences).  With equal tiles the runs are
+        # bit-identical (tests/test_gpu_qspace.py).
         out.update({"per_chain_value": n_l / wall1, "per_chain_wall_s": wall1,
                     "per_chain_tiles": list(tiles1),
+                    "per_chain_vs_batched_max_rel_diff_a_first50": worst50,
                     "per_chain_vs_batched_max_rel_diff_a": worst,
                     "batched_speedup": wall1 / wall})
     eng.close()

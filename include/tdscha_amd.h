@@ -217,6 +217,14 @@ int tdscha_qspace_lanczos_run(tdscha_qctx* ctx, int n_chains, const int* iq_pert
  */
 int tdscha_qspace_set_tiles(tdscha_qctx* ctx, int weights_tile, int d2v_chunk, int reorth_jtile);
 int tdscha_qspace_get_tiles(const tdscha_qctx* ctx, int* tiles_out);
+/*
+ * Form of the d2v kernel: 1 (default) stages configuration slabs of every
+ * q-point in LDS and gives each thread one (chain, pair) block; 0 gives each
+ * workgroup one (chain, pair) block and each thread a configuration.  Grids
+ * too large for a one-configuration slab use form 0 whatever is set.  Both
+ * compute the same sums in different orders.
+ */
+int tdscha_qspace_set_d2v_form(tdscha_qctx* ctx, int form);
 int tdscha_qspace_autotune(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
                            const int* q_pair_map, int reps, int* tiles_out, double* best_ms);
 

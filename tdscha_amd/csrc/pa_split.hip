@@ -110,28 +110,6 @@ __device__ __forceinline__ int frag_off(int b, int lane) {
   return (lane >> 5) * (kBR * 8) + (32 * b + (lane & 31)) * 8;
 }
 
-// direct global -> LDS copy of 16 bytes per lane: lane i's bytes land at
-// lds_base + 16 i (lds_base is wave-uniform)
-typedef __attribute__((address_space(3))) void* lds_ptr_t;
-typedef __attribute__((address_space(1))) void* gbl_ptr_t;
-__device__ __forceinline__ void dma16(const void* g, void* lds_base) {
-  __builtin_amdgcn_global_load_lds((gbl_ptr_t)g, (lds_ptr_t)lds_base, 16, 0, 0);
-}
-
-// s_waitcnt vmcnt(N) (expcnt and lgkmcnt not waited), gfx9 encoding
-#define TDS_WAIT_VMCNT(N) \
-  __builtin_amdgcn_s_waitcnt(((N) & 15) | (((N) >> 4) << 14) | 0x0F70)
-
-// Work-group barrier without a memory fence.  The direct-to-LDS copies are
-// completed explicitly (vmcnt) before it; this waits for the wave's own LDS
-// reads and writes (lgkmcnt(0)), and the empty asm statements keep the
-// compiler from moving memory accesses across it.
-__device__ __forceinline__ void lds_barrier() {
-  __builtin_amdgcn_s_waitcnt(0xC07F);
-  asm volatile("" ::: "memory");
-  __builtin_amdgcn_s_barrier();
-  asm volatile("" ::: "memory");
-}
 
 // ------------------------------------------------------------ static prep
 __global__ void colmax_kernel(const double* __restrict__ U, const double* __restrict__ F, long Mp,

@@ -39,6 +39,10 @@ namespace tdscha {
 
 constexpr int kQMaxChains = 256;
 constexpr int kQMaxQ = 512;
+constexpr int kQWeightsChains = 2;   // chains per q_weights workgroup
+constexpr int kD2vChains = 16;       // chains per slab q_d2v workgroup
+constexpr int kSlabBytes = 65536;    // LDS of one slab q_d2v workgroup (two buffers)
+constexpr int kD2vThreads = 256;     // threads (items) per slab q_d2v workgroup
 
 struct QCtx {
   int nq = 0, nb = 0, N = 0;
@@ -52,6 +56,10 @@ struct QCtx {
   int* valid = nullptr;     // [nq][nb]
   double* fY = nullptr;     // [nq][nb]
   double* fP = nullptr;     // [nq][nb]
+  double2* S = nullptr;     // slab images of (fY X, Y) for q_d2v_slab
+  int slabT = 0;            // configurations per slab (0: no slab images)
+  int d2v_form = 1;         // 1: q_d2v_slab when slab images exist, 0: q_d2v
+  int n_cu = 256;           // compute units of the context's device
   // tile choices (autotuner)
   int weights_tile = 128;   // configurations per workgroup of q_weights
   int d2v_chunk = 0;        // configurations per workgroup of q_d2v (0 = automatic)
@@ -61,7 +69,7 @@ struct QCtx {
 
 // Chain tables and work buffers of one batch.
 struct QBatch {
-  int C = 0, MP = 0, ld = 0, nch = 1, ntile = 1;
+  int C = 0, MP = 0, maxp = 0, ld = 0, nch = 1, ntile = 1;
   std::vector<int> h_len, h_np;
   int* iqp = nullptr;       // [C]
   int* npairs = nullptr;    // [C]
@@ -73,12 +81,17 @@ struct QBatch {
   int* apair = nullptr;     // [C][nq]
   int* aorient = nullptr;   // [C][nq]
   int* active = nullptr;    // [C]
-  double2* alpha = nullptr; // [C][MP][nb*nb]
+  double2* alpha = nullptr; // [C][nq][nb*nb] oriented alpha1 block of each q
   double2* R1 = nullptr;    // [C][nb]
   double2* wc = nullptr;    // [C][N]
   double2* wd = nullptr;
   double2* fpart = nullptr; // [C][ntile][nb]
   double2* dpart = nullptr; // [C][MP][nch][nb*nb]
+  // slab form of q_d2v: (chain, pair, q1, q2) items sorted by chain, cut into
+  // workgroups of at most kD2vThreads items and kD2vChains chains
+  int slabT = 0, chunk = 0, nib = 0, n_items = 0;
+  int4* items = nullptr;    // [n_items]
+  int* blk_start = nullptr; // [nib + 1]
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
@@ -92,6 +105,21 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) {
   return make_double2(a.x + b.x, a.y + b.y);
 }
 __device__ __forceinline__ double2 csc(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+// acc + a b and acc + a conj(b) as four FMAs
+__device__ __forceinline__ double2 cmac(double2 acc, double2 a, double2 b) {
+  acc.x = __builtin_fma(a.x, b.x, acc.x);
+  acc.x = __builtin_fma(-a.y, b.y, acc.x);
+  acc.y = __builtin_fma(a.x, b.y, acc.y);
+  acc.y = __builtin_fma(a.y, b.x, acc.y);
+  return acc;
+}
+__device__ __forceinline__ double2 cmacj(double2 acc, double2 a, double2 b) {
+  acc.x = __builtin_fma(a.x, b.x, acc.x);
+  acc.x = __builtin_fma(a.y, b.y, acc.x);
+  acc.y = __builtin_fma(a.y, b.x, acc.y);
+  acc.y = __builtin_fma(-a.x, b.y, acc.y);
+  return acc;
+}
 
 __device__ __forceinline__ int tri_index(int i, int j, int nb) {   // i <= j
   return i * nb - i * (i - 1) / 2 + (j - i);
@@ -128,19 +156,24 @@ __device__ __forceinline__ PairFac pair_factors(double w1, double w2, double n1,
   return f;
 }
 
+// alpha1 block of the pair that q belongs to, oriented for q (JQ:519-526):
+// alq[c][q][v1][v2] = alpha[v1][v2] when q is the pair's first q-point,
+// conj(alpha[v2][v1]) when it is the second.  One workgroup per (q, chain).
 template <int NB>
 __global__ void q_unpack(const double2* __restrict__ psi, int ld, const int* __restrict__ active,
-                         const int* __restrict__ npairs, const int* __restrict__ pq1,
-                         const int* __restrict__ pq2, const int* __restrict__ offa,
-                         const int* __restrict__ offb, int MP, const double* __restrict__ wq,
-                         const double* __restrict__ nbose, const int* __restrict__ valid,
-                         int ignore_v3, double2* __restrict__ alpha, double2* __restrict__ R1) {
-  const int c = blockIdx.y, p = blockIdx.x, t = threadIdx.x;
+                         const int* __restrict__ pq1, const int* __restrict__ pq2,
+                         const int* __restrict__ offa, const int* __restrict__ offb, int MP,
+                         int nq, const int* __restrict__ apair, const int* __restrict__ aorient,
+                         const double* __restrict__ wq, const double* __restrict__ nbose,
+                         const int* __restrict__ valid, int ignore_v3,
+                         double2* __restrict__ alq, double2* __restrict__ R1) {
+  const int c = blockIdx.y, q = blockIdx.x, t = threadIdx.x;
   if (!active[c]) return;
   const double2* ps = psi + (long)c * ld;
-  if (p == 0 && t < NB) R1[c * NB + t] = ignore_v3 ? make_double2(0.0, 0.0) : ps[t];
-  if (p >= npairs[c] || t >= NB * NB) return;
-  const int i = t / NB, j = t % NB;
+  if (q == 0 && t < NB) R1[c * NB + t] = ignore_v3 ? make_double2(0.0, 0.0) : ps[t];
+  if (t >= NB * NB) return;
+  const int p = apair[c * nq + q], o = aorient[c * nq + q];
+  const int i = o ? t % NB : t / NB, j = o ? t / NB : t % NB;
   const int q1 = pq1[c * MP + p], q2 = pq2[c * MP + p];
   const int full = q1 < q2;
   const double2 a = block_entry(ps, offa[c * MP + p], full, i, j, NB);
@@ -148,108 +181,8 @@ __global__ void q_unpack(const double2* __restrict__ psi, int ld, const int* __r
   const PairFac f = pair_factors(wq[q1 * NB + i], wq[q2 * NB + j], nbose[q1 * NB + i],
                                  nbose[q2 * NB + j], valid[q1 * NB + i], valid[q2 * NB + j]);
   const double ca = f.w2x * sqrt(-0.5 * f.chim), cb = f.w2x * sqrt(0.5 * f.chip);
-  alpha[((long)c * MP + p) * NB * NB + t] =
-      make_double2(ca * a.x - cb * b.x, ca * a.y - cb * b.y);
-}
-
-// One thread per configuration; blockIdx.x = chain so that the blocks that
-// read one configuration tile run back to back.
-template <int NB>
-__global__ void __launch_bounds__(256) q_weights(
-    const double2* __restrict__ X, const double2* __restrict__ Y, const double* __restrict__ rho,
-    int N, int nq, const double* __restrict__ fY, const double* __restrict__ fP,
-    const int* __restrict__ active, const int* __restrict__ iqp, const int* __restrict__ npairs,
-    const int* __restrict__ partner, const int* __restrict__ apair,
-    const int* __restrict__ aorient, const double2* __restrict__ alpha,
-    const double2* __restrict__ R1, int MP, int apply_v4, double2* __restrict__ wc,
-    double2* __restrict__ wd, double2* __restrict__ fpart, int ntile) {
-  extern __shared__ double smem[];
-  const int c = blockIdx.x, tile = blockIdx.y, t = threadIdx.x;
-  if (!active[c]) return;
-  const int np = npairs[c];
-  double2* al = reinterpret_cast<double2*>(smem);                      // [np][NB*NB]
-  double* fYs = smem + 2 * np * NB * NB;                                // [nq][NB]
-  double* fPs = fYs + nq * NB;
-  int* tabs = reinterpret_cast<int*>(fPs + nq * NB);                    // partner, apair, orient
-  for (int e = t; e < np * NB * NB; e += blockDim.x) al[e] = alpha[(long)c * MP * NB * NB + e];
-  for (int e = t; e < nq * NB; e += blockDim.x) {
-    fYs[e] = fY[e];
-    fPs[e] = fP[e];
-  }
-  for (int e = t; e < nq; e += blockDim.x) {
-    tabs[e] = partner[c * nq + e];
-    tabs[nq + e] = apair[c * nq + e];
-    tabs[2 * nq + e] = aorient[c * nq + e];
-  }
-  __syncthreads();
-  const int i = tile * blockDim.x + t;
-  double2 fc[NB];
-#pragma unroll
-  for (int v = 0; v < NB; ++v) fc[v] = make_double2(0.0, 0.0);
-  if (i < N) {
-    double2 tot = make_double2(0.0, 0.0), bfw = make_double2(0.0, 0.0);
-    for (int q = 0; q < nq; ++q) {
-      const int q2 = tabs[q], p = tabs[nq + q], o = tabs[2 * nq + q];
-      const double2* A = al + p * NB * NB;
-      double2 x2[NB];
-#pragma unroll
-      for (int v = 0; v < NB; ++v) x2[v] = X[((long)q2 * NB + v) * N + i];
-#pragma unroll
-      for (int v1 = 0; v1 < NB; ++v1) {
-        double2 bu = make_double2(0.0, 0.0);
-#pragma unroll
-        for (int v2 = 0; v2 < NB; ++v2) {
-          // orientation 0: alpha[v1][v2]; 1: conj(alpha[v2][v1]) (the pair's
-          // second q-point, JQ:519-526)
-          const double2 a = o ? cj(A[v2 * NB + v1]) : A[v1 * NB + v2];
-          bu = cadd(bu, cmul(a, x2[v2]));
-        }
-        const double2 xq = X[((long)q * NB + v1) * N + i];
-        const double2 yq = Y[((long)q * NB + v1) * N + i];
-        tot = cadd(tot, cmulcj(bu, xq));                 // conj(x) bu
-        bfw = cadd(bfw, csc(cmulcj(yq, bu), fPs[q * NB + v1]));   // conj(bu) fpsi y
-      }
-    }
-    const int qp = iqp[c];
-    const double r = rho[i];
-    double2 wR = make_double2(0.0, 0.0), wRf = make_double2(0.0, 0.0);
-    double2 xp[NB], yp[NB];
-#pragma unroll
-    for (int v = 0; v < NB; ++v) {
-      xp[v] = X[((long)qp * NB + v) * N + i];
-      yp[v] = Y[((long)qp * NB + v) * N + i];
-      const double2 r1 = R1[c * NB + v];
-      wR = cadd(wR, csc(cmulcj(xp[v], r1), fYs[qp * NB + v]));
-      wRf = cadd(wRf, cmulcj(yp[v], r1));
-    }
-    wR = csc(wR, r / 3.0);
-    wRf = csc(wRf, r / 3.0);
-    const double2 w1 = csc(tot, -0.5 * r / 3.0);
-    const double2 w2 = csc(bfw, -r / 3.0);
-#pragma unroll
-    for (int v = 0; v < NB; ++v)
-      fc[v] = cadd(cmul(w1, yp[v]), csc(cmul(w2, xp[v]), fYs[qp * NB + v]));
-    double2 c1 = wR, c2 = wRf;
-    if (apply_v4) {
-      c1 = cadd(c1, csc(tot, -r / 8.0));
-      c2 = cadd(c2, csc(bfw, -r / 4.0));
-    }
-    wc[(long)c * N + i] = c1;
-    wd[(long)c * N + i] = c2;
-  }
-  // fixed-order block reduction of the f_pert contributions
-  __syncthreads();
-  double2* red = reinterpret_cast<double2*>(smem);
-  for (int v = 0; v < NB; ++v) {
-    red[t] = fc[v];
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if (t < s) red[t] = cadd(red[t], red[t + s]);
-      __syncthreads();
-    }
-    if (t == 0) fpart[((long)c * ntile + tile) * NB + v] = red[0];
-    __syncthreads();
-  }
+  const double2 al = make_double2(ca * a.x - cb * b.x, ca * a.y - cb * b.y);
+  alq[((long)c * nq + q) * NB * NB + t] = o ? cj(al) : al;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -258,8 +191,138 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Workgroup b of a tiled launch runs on XCD b % 8.  This maps b to (tile,
+// group) so that every group of one tile runs on the same XCD, tiles
+// t = 8 s + x going to XCD x in order: the groups of a tile then read its
+// data through one L2.  Returns false for the padding workgroups (tile >=
+// ntile) of the last round.
+__device__ __forceinline__ bool xcd_tile(int b, int ngroups, int ntile, int& tile, int& group) {
+  const int s = b >> 3;
+  group = s % ngroups;
+  tile = (s / ngroups) * 8 + (b & 7);
+  return tile < ntile;
+}
+
+// One thread per configuration and CB chains per workgroup: the own-q
+// vectors x_q and fP_q y_q are loaded once for the CB chains, each chain
+// then loads its partner x_{q_pert - q} and applies its oriented alpha block,
+// which is the same for every thread (scalar loads).
+template <int NB, int CB>
+__global__ void __launch_bounds__(256) q_weights(
+    const double2* __restrict__ X, const double2* __restrict__ Y, const double* __restrict__ rho,
+    int N, int nq, const double* __restrict__ fY, const double* __restrict__ fP,
+    const int* __restrict__ active, const int* __restrict__ iqp,
+    const int* __restrict__ partner, const double2* __restrict__ alq,
+    const double2* __restrict__ R1, int C, int ngroups, int ntile, int apply_v4,
+    double2* __restrict__ wc, double2* __restrict__ wd, double2* __restrict__ fpart) {
+  __shared__ double red[4][CB][2 * NB];
+  int tile, g;
+  if (!xcd_tile(blockIdx.x, ngroups, ntile, tile, g)) return;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int c0 = g * CB;
+  unsigned act = 0;
+#pragma unroll
+  for (int k = 0; k < CB; ++k)
+    if (c0 + k < C && active[c0 + k]) act |= 1u << k;
+  if (!act) return;
+  const int i = tile * blockDim.x + t;
+  const bool in = i < N;
+  double2 tot[CB], bfw[CB];
+#pragma unroll
+  for (int k = 0; k < CB; ++k) tot[k] = bfw[k] = make_double2(0.0, 0.0);
+  if (in) {
+    for (int q = 0; q < nq; ++q) {
+      double2 xq[NB], yq[NB];
+#pragma unroll
+      for (int v = 0; v < NB; ++v) {
+        xq[v] = X[((long)q * NB + v) * N + i];
+        yq[v] = csc(Y[((long)q * NB + v) * N + i], fP[q * NB + v]);
+      }
+#pragma unroll
+      for (int k = 0; k < CB; ++k) {
+        if (!((act >> k) & 1u)) continue;
+        const int c = c0 + k;
+        const int q2 = partner[c * nq + q];
+        const double2* A = alq + ((long)c * nq + q) * NB * NB;
+        double2 x2[NB];
+#pragma unroll
+        for (int v = 0; v < NB; ++v) x2[v] = X[((long)q2 * NB + v) * N + i];
+#pragma unroll
+        for (int v1 = 0; v1 < NB; ++v1) {
+          double2 bu = cmul(A[v1 * NB], x2[0]);
+#pragma unroll
+          for (int v2 = 1; v2 < NB; ++v2) bu = cmac(bu, A[v1 * NB + v2], x2[v2]);
+          tot[k] = cmacj(tot[k], bu, xq[v1]);      // conj(x) bu
+          bfw[k] = cmacj(bfw[k], yq[v1], bu);      // conj(bu) fP y
+        }
+      }
+    }
+  }
+  const double r = in ? rho[i] : 0.0;
+#pragma unroll
+  for (int k = 0; k < CB; ++k) {
+    if (!((act >> k) & 1u)) continue;
+    const int c = c0 + k;
+    const int qp = iqp[c];
+    double2 fc[NB];
+#pragma unroll
+    for (int v = 0; v < NB; ++v) fc[v] = make_double2(0.0, 0.0);
+    if (in) {
+      double2 wR = make_double2(0.0, 0.0), wRf = make_double2(0.0, 0.0);
+      double2 xp[NB], yp[NB];
+#pragma unroll
+      for (int v = 0; v < NB; ++v) {
+        xp[v] = X[((long)qp * NB + v) * N + i];
+        yp[v] = Y[((long)qp * NB + v) * N + i];
+        const double2 r1 = R1[c * NB + v];
+        wR = cadd(wR, csc(cmulcj(xp[v], r1), fY[qp * NB + v]));
+        wRf = cadd(wRf, cmulcj(yp[v], r1));
+      }
+      wR = csc(wR, r / 3.0);
+      wRf = csc(wRf, r / 3.0);
+      const double2 w1 = csc(tot[k], -0.5 * r / 3.0);
+      const double2 w2 = csc(bfw[k], -r / 3.0);
+#pragma unroll
+      for (int v = 0; v < NB; ++v)
+        fc[v] = cadd(cmul(w1, yp[v]), csc(cmul(w2, xp[v]), fY[qp * NB + v]));
+      double2 c1 = wR, c2 = wRf;
+      if (apply_v4) {
+        c1 = cadd(c1, csc(tot[k], -r / 8.0));
+        c2 = cadd(c2, csc(bfw[k], -r / 4.0));
+      }
+      wc[(long)c * N + i] = c1;
+      wd[(long)c * N + i] = c2;
+    }
+    // fixed-order reduction of the f_pert contributions: waves, then the
+    // workgroup's waves in order
+#pragma unroll
+    for (int v = 0; v < NB; ++v) {
+      const double sx = wave_sum(fc[v].x), sy = wave_sum(fc[v].y);
+      if (lane == 0) {
+        red[wv][k][2 * v] = sx;
+        red[wv][k][2 * v + 1] = sy;
+      }
+    }
+  }
+  __syncthreads();
+  if (t < CB * NB) {
+    const int k = t / NB, v = t % NB;
+    if ((act >> k) & 1u) {
+      double sx = 0.0, sy = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+        sx += red[w][k][2 * v];
+        sy += red[w][k][2 * v + 1];
+      }
+      fpart[((long)(c0 + k) * ntile + tile) * NB + v] = make_double2(sx, sy);
+    }
+  }
+}
+
+// grid (largest pair count, chains, configuration chunks); the 36 complex
+// accumulators take 144 VGPRs, the cap of two waves per SIMD leaves the rest
+// for the loads of one configuration
 template <int NB>
-__global__ void __launch_bounds__(256) q_d2v(
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) q_d2v(
     const double2* __restrict__ X, const double2* __restrict__ Y, int N,
     const double* __restrict__ fY, const int* __restrict__ active,
     const int* __restrict__ npairs, const int* __restrict__ pq1, const int* __restrict__ pq2,
@@ -291,24 +354,23 @@ __global__ void __launch_bounds__(256) q_d2v(
       r21[v] = Y[((long)q1 * NB + v) * N + i];
       const double2 r12 = csc(X[((long)q2 * NB + v) * N + i], f2[v]);
       const double2 r22 = Y[((long)q2 * NB + v) * N + i];
-      // Q1 = -conj(wc) r22 - conj(wd) r12 ; Q2 = -conj(wc) r12
-      const double2 m1 = cmul(cj(wci), r22), m2 = cmul(cj(wdi), r12);
-      Q1[v] = make_double2(-m1.x - m2.x, -m1.y - m2.y);
-      const double2 m3 = cmul(cj(wci), r12);
-      Q2[v] = make_double2(-m3.x, -m3.y);
+      // -Q1 = conj(wc) r22 + conj(wd) r12 ; -Q2 = conj(wc) r12
+      Q1[v] = cmac(cmul(cj(wci), r22), cj(wdi), r12);
+      Q2[v] = cmul(cj(wci), r12);
     }
+    // accumulates -(r11 Q1^H + r21 Q2^H); the sign is restored below
 #pragma unroll
     for (int a = 0; a < NB; ++a)
 #pragma unroll
       for (int b = 0; b < NB; ++b)
-        acc[a][b] = cadd(acc[a][b], cadd(cmulcj(r11[a], Q1[b]), cmulcj(r21[a], Q2[b])));
+        acc[a][b] = cmacj(cmacj(acc[a][b], r11[a], Q1[b]), r21[a], Q2[b]);
   }
   const int lane = t & 63, wv = t >> 6;
 #pragma unroll
   for (int a = 0; a < NB; ++a)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      const double sx = wave_sum(acc[a][b].x), sy = wave_sum(acc[a][b].y);
+      const double sx = -wave_sum(acc[a][b].x), sy = -wave_sum(acc[a][b].y);
       if (lane == 0) {
         red[wv][2 * (a * NB + b)] = sx;
         red[wv][2 * (a * NB + b) + 1] = sy;
@@ -323,6 +385,138 @@ __global__ void __launch_bounds__(256) q_d2v(
       sy += red[w][2 * t + 1];
     }
     dpart[(((long)c * MP + p) * nch + ch) * NB * NB + t] = make_double2(sx, sy);
+  }
+}
+
+// Slab images: the configurations [s T, s T + T) of slab s as one
+// contiguous block of U = 2 nq SR entries, SR = NB T + 1: entry
+// (h nq + q) SR + v T + j holds fY[q][v] x_{q,v} (h = 0) or y_{q,v} (h = 1)
+// of configuration s T + j, zero past N; the last entry of each row is a
+// zero pad that puts the rows of neighbouring q-points on different LDS
+// banks.  One thread per entry.
+template <int NB>
+__global__ void q_slab_build(const double2* __restrict__ X, const double2* __restrict__ Y,
+                             const double* __restrict__ fY, int N, int nq, int lgT, long total,
+                             double2* __restrict__ S) {
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int T = 1 << lgT, SR = NB * T + 1;
+  const long U = 2L * nq * SR;
+  const long sl = e / U;
+  const int u = (int)(e - sl * U);
+  const int hq = u / SR, w = u - hq * SR;
+  const int h = hq >= nq, q = h ? hq - nq : hq;
+  double2 val = make_double2(0.0, 0.0);
+  if (w < NB * T) {
+    const int v = w >> lgT, j = w & (T - 1);
+    const long cfg = sl * T + j;
+    if (cfg < N) {
+      const long r = (long)q * NB + v;
+      val = h ? Y[r * N + cfg] : csc(X[r * N + cfg], fY[r]);
+    }
+  }
+  S[e] = val;
+}
+
+// d2v blocks from slab images in LDS.  Workgroup (item block ib,
+// configuration chunk ch) copies the slab images of its chunk, and the
+// weights wc, wd of its chains for the slab's configurations, into LDS with direct global -> LDS loads into two
+// buffers in turn: the copy of slab k + 1 runs while the threads accumulate
+// slab k.  The two buffers are distinct arrays and each half of the unrolled
+// loop names its own, so that the compiler sees that the reads of one do not
+// depend on the copies into the other.  Each thread accumulates the 6x6
+// block of one (chain, pair) item, in configuration order.  The item blocks
+// of one chunk run on one XCD and share the chunk through its L2.
+constexpr int kSlabUnits = kSlabBytes / 2 / (int)sizeof(double2);
+
+template <int NB>
+__global__ void __launch_bounds__(kD2vThreads) __attribute__((amdgpu_waves_per_eu(2))) q_d2v_slab(
+    const double2* __restrict__ S, int N, int nq, const int* __restrict__ active,
+    const int4* __restrict__ items, const int* __restrict__ blk_start, int nib, int MP,
+    const double2* __restrict__ wc, const double2* __restrict__ wd, int lgT, int chunk, int nch,
+    double2* __restrict__ dpart) {
+  __shared__ double2 bufA[kSlabUnits], bufB[kSlabUnits];
+  int ch, ib;
+  if (!xcd_tile(blockIdx.x, nib, nch, ch, ib)) return;
+  const int T = 1 << lgT, SR = NB * T + 1, U = 2 * nq * SR;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6, nwave = blockDim.x >> 6;
+  const int i0 = blk_start[ib], i1 = blk_start[ib + 1];
+  const int c_lo = items[i0].x, ncb = items[i1 - 1].x - c_lo + 1;
+  bool any = false;
+  for (int k = 0; k < ncb; ++k) any |= active[c_lo + k] != 0;
+  if (!any) return;
+  int4 it = make_int4(0, 0, 0, 0);
+  bool mine = i0 + t < i1;
+  if (mine) {
+    it = items[i0 + t];
+    mine = active[it.x] != 0;
+  }
+  const int cw = (it.x - c_lo) << (lgT + 1);
+  const int nW = ncb << (lgT + 1);
+  const int lo = ch * chunk, hi = min(N, lo + chunk);
+  auto issue = [&](int s0, double2* dst) {
+    const long sl = s0 >> lgT;
+    const double2* src = S + sl * U;
+    for (int k = wave; k * 64 < U; k += nwave)
+      if (k * 64 + lane < U) dma16(src + k * 64 + lane, dst + k * 64);
+    // weights: entry (chain k, c1 | c2, configuration j) at (2 k + h) T + j;
+    // past N any finite value (the image is zero there)
+    for (int k = wave; k * 64 < nW; k += nwave) {
+      const int e = k * 64 + lane;
+      if (e < nW) {
+        const int j = e & (T - 1), h = (e >> lgT) & 1, c = c_lo + (e >> (lgT + 1));
+        dma16((h ? wd : wc) + (long)c * N + min(s0 + j, N - 1), dst + U + k * 64);
+      }
+    }
+  };
+  double2 acc[NB][NB];
+#pragma unroll
+  for (int a = 0; a < NB; ++a)
+#pragma unroll
+    for (int b = 0; b < NB; ++b) acc[a][b] = make_double2(0.0, 0.0);
+  auto accumulate = [&](const double2* img) {
+    if (!mine) return;
+    for (int j = 0; j < T; ++j) {
+      const double2 wci = img[U + cw + j], wdi = img[U + cw + T + j];
+      const double2* x1 = img + it.z * SR + j;
+      const double2* y1 = img + (nq + it.z) * SR + j;
+      const double2* x2 = img + it.w * SR + j;
+      const double2* y2 = img + (nq + it.w) * SR + j;
+      double2 Q1[NB], Q2[NB];
+#pragma unroll
+      for (int v = 0; v < NB; ++v) {
+        // -Q1 = conj(wc) r22 + conj(wd) r12, -Q2 = conj(wc) r12 with
+        // r12 = fY x2 (the image holds fY x) and r22 = y2
+        const double2 r12 = x2[v * T], r22 = y2[v * T];
+        Q1[v] = cmac(cmul(cj(wci), r22), cj(wdi), r12);
+        Q2[v] = cmul(cj(wci), r12);
+      }
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        const double2 r11 = x1[a * T], r21 = y1[a * T];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[a][b] = cmacj(cmacj(acc[a][b], r11, Q1[b]), r21, Q2[b]);
+      }
+    }
+  };
+  issue(lo, bufA);
+  for (int s0 = lo; s0 < hi; s0 += 2 * T) {
+    TDS_WAIT_VMCNT(0);
+    lds_barrier();                     // slab s0 in bufA; bufB free
+    if (s0 + T < hi) issue(s0 + T, bufB);
+    accumulate(bufA);
+    if (s0 + T >= hi) break;
+    TDS_WAIT_VMCNT(0);
+    lds_barrier();                     // slab s0 + T in bufB; bufA free
+    if (s0 + 2 * T < hi) issue(s0 + 2 * T, bufA);
+    accumulate(bufB);
+  }
+  if (mine) {
+    double2* o = dpart + (((long)it.x * MP + it.y) * nch + ch) * NB * NB;
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) o[a * NB + b] = make_double2(-acc[a][b].x, -acc[a][b].y);
   }
 }
 
@@ -421,7 +615,8 @@ __global__ void q_expand(const double2* __restrict__ X, const double2* __restric
 // ------------------------------------------------------------ batch setup
 static void free_batch(QBatch& b) {
   void* p[] = {b.iqp,   b.npairs, b.pq1, b.pq2, b.offa, b.offb,  b.partner, b.apair,
-               b.aorient, b.active, b.alpha, b.R1, b.wc,  b.wd,   b.fpart, b.dpart};
+               b.aorient, b.active, b.alpha, b.R1, b.wc,  b.wd,   b.fpart, b.dpart,
+               b.items, b.blk_start};
   for (void* x : p)
     if (x) (void)hipFree(x);
   b = QBatch();
@@ -434,6 +629,65 @@ static int choose_chunk(const QCtx* q, int C, int MP_used) {
   long nch = std::max(1L, 2048 / std::max(1L, blocks));
   int chunk = (int)((q->N + nch - 1) / nch);
   return std::max(chunk, std::min(q->N, 1024));
+}
+
+// Configurations per slab of the slab q_d2v: the largest power of two up to
+// 16 whose slab image and the weights of kD2vChains chains fit one of its
+// two LDS buffers (kSlabBytes / 2); 0 when one configuration does not fit.
+static size_t slab_units(int nq, int nb, int T) {
+  return 2 * (size_t)nq * (nb * T + 1) + 2 * (size_t)kD2vChains * T;
+}
+
+static int log2_pow2(int T) {
+  int lg = 0;
+  while ((1 << lg) < T) ++lg;
+  return lg;
+}
+
+static int slab_configs(int nq, int nb) {
+  int T = 16;
+  while (T > 0 && slab_units(nq, nb, T) * sizeof(double2) > (size_t)kSlabBytes / 2) T >>= 1;
+  return T;
+}
+
+// (Re)builds the slab images of the context's ensemble; without room for
+// one configuration per slab the context keeps none and q_d2v runs instead.
+static int build_slabs(QCtx* q) {
+  if (q->S) (void)hipFree(q->S);
+  q->S = nullptr;
+  q->slabT = slab_configs(q->nq, q->nb);
+  if (!q->slabT) return 0;
+  const int lg = log2_pow2(q->slabT);
+  const long nslab = (q->N + q->slabT - 1) / q->slabT;
+  const long total = nslab * 2L * q->nq * (q->nb * q->slabT + 1);
+  if (hipMalloc((void**)&q->S, sizeof(double2) * (size_t)total) != hipSuccess) {
+    q->S = nullptr;
+    set_error("q-space: allocation of the slab images failed");
+    return -1;
+  }
+  const unsigned nblk = (unsigned)((total + 255) / 256);
+  if (q->nb == 6)
+    hipLaunchKernelGGL(q_slab_build<6>, dim3(nblk), dim3(256), 0, q->stream, q->X, q->Y, q->fY,
+                       q->N, q->nq, lg, total, q->S);
+  else
+    hipLaunchKernelGGL(q_slab_build<3>, dim3(nblk), dim3(256), 0, q->stream, q->X, q->Y, q->fY,
+                       q->N, q->nq, lg, total, q->S);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(q->stream) != hipSuccess) {
+    set_error("q-space: slab image kernel failed");
+    return -1;
+  }
+  return 0;
+}
+
+// d2v chunk (configurations per workgroup) that gives `rounds` rounds of
+// slab workgroups on the device (n_item_blocks workgroups per chunk, two
+// resident workgroups per CU: two buffers of half the CU's LDS and 256
+// threads at up to 256 VGPRs), at least 8 slabs.
+static int slab_chunk(const QCtx* q, int n_item_blocks, int rounds) {
+  const long slots = 2L * q->n_cu * rounds;
+  const long nch = std::max(1L, slots / std::max(1, n_item_blocks));
+  const long chunk = std::max<long>(8L * q->slabT, (q->N + nch - 1) / nch);
+  return (int)((chunk + q->slabT - 1) / q->slabT * q->slabT);
 }
 
 // Host tables from the pair maps; ld_min forces a minimum stride.
@@ -481,17 +735,49 @@ static int build_batch(QCtx* q, int C, const int* iq_pert, const int* pair_map, 
     h_len[c] = off;
     maxp = std::max(maxp, np);
   }
-  const size_t lds = sizeof(double) * (2 * (size_t)maxp * nb * nb + 2 * (size_t)nq * nb) +
-                     sizeof(int) * 3 * (size_t)nq;
-  TDS_REQUIRE(lds <= 65536, 2, "q-space: pair tables of this grid exceed the workgroup LDS");
   int ld = ld_min;
   for (int c = 0; c < C; ++c) ld = std::max(ld, h_len[c]);
   b.ld = ld;
   b.h_len = h_len;
   b.h_np = h_np;
   b.ntile = (q->N + q->weights_tile - 1) / q->weights_tile;
-  const int chunk = choose_chunk(q, C, maxp);
+  b.maxp = maxp;
+  // slab items and their workgroups
+  std::vector<int4> h_items;
+  std::vector<int> h_bs(1, 0);
+  b.slabT = q->S && q->d2v_form ? q->slabT : 0;
+  if (b.slabT) {
+    // items sorted by chain, cut at `cap` items or kD2vChains chains
+    auto partition = [&](int cap) {
+      h_items.clear();
+      h_bs.assign(1, 0);
+      int first_c = -1;
+      for (int c = 0; c < C; ++c)
+        for (int p = 0; p < h_np[c]; ++p) {
+          const int cur = (int)h_items.size() - h_bs.back();
+          if (cur == cap || (cur > 0 && c - first_c >= kD2vChains))
+            h_bs.push_back((int)h_items.size());
+          if ((int)h_items.size() == h_bs.back()) first_c = c;
+          h_items.push_back(make_int4(c, p, h_pq1[(size_t)c * nq + p], h_pq2[(size_t)c * nq + p]));
+        }
+      h_bs.push_back((int)h_items.size());
+    };
+    partition(kD2vThreads);
+    const int nfull = (int)h_bs.size() - 1;      // then the same count of even workgroups
+    partition(((int)h_items.size() + nfull - 1) / nfull);
+    b.n_items = (int)h_items.size();
+    b.nib = (int)h_bs.size() - 1;
+  }
+  int chunk;
+  if (b.slabT) {
+    // automatic: one round of workgroups on the device's CUs
+    chunk = q->d2v_chunk > 0 ? q->d2v_chunk : slab_chunk(q, b.nib, 1);
+    chunk = (chunk + b.slabT - 1) / b.slabT * b.slabT;
+  } else {
+    chunk = choose_chunk(q, C, maxp);
+  }
   b.nch = (q->N + chunk - 1) / chunk;
+  b.chunk = chunk;
   const size_t nbb = (size_t)nb * nb;
   bool ok = hipMalloc((void**)&b.iqp, sizeof(int) * C) == hipSuccess &&
             hipMalloc((void**)&b.npairs, sizeof(int) * C) == hipSuccess &&
@@ -508,7 +794,10 @@ static int build_batch(QCtx* q, int C, const int* iq_pert, const int* pair_map, 
             hipMalloc((void**)&b.wc, sizeof(double2) * (size_t)C * q->N) == hipSuccess &&
             hipMalloc((void**)&b.wd, sizeof(double2) * (size_t)C * q->N) == hipSuccess &&
             hipMalloc((void**)&b.fpart, sizeof(double2) * (size_t)C * b.ntile * nb) == hipSuccess &&
-            hipMalloc((void**)&b.dpart, sizeof(double2) * (size_t)C * nq * b.nch * nbb) == hipSuccess;
+            hipMalloc((void**)&b.dpart, sizeof(double2) * (size_t)C * nq * b.nch * nbb) == hipSuccess &&
+            (!b.slabT ||
+             (hipMalloc((void**)&b.items, sizeof(int4) * (size_t)b.n_items) == hipSuccess &&
+              hipMalloc((void**)&b.blk_start, sizeof(int) * (size_t)(b.nib + 1)) == hipSuccess));
   if (!ok) {
     free_batch(b);
     set_error("q-space: device allocation failed");
@@ -524,7 +813,12 @@ static int build_batch(QCtx* q, int C, const int* iq_pert, const int* pair_map, 
        hipMemcpy(b.partner, h_part.data(), sizeof(int) * C * nq, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(b.apair, h_apair.data(), sizeof(int) * C * nq, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(b.aorient, h_orient.data(), sizeof(int) * C * nq, hipMemcpyHostToDevice) == hipSuccess &&
-       hipMemcpy(b.active, ones.data(), sizeof(int) * C, hipMemcpyHostToDevice) == hipSuccess;
+       hipMemcpy(b.active, ones.data(), sizeof(int) * C, hipMemcpyHostToDevice) == hipSuccess &&
+       (!b.slabT ||
+        (hipMemcpy(b.items, h_items.data(), sizeof(int4) * (size_t)b.n_items,
+                   hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(b.blk_start, h_bs.data(), sizeof(int) * (size_t)(b.nib + 1),
+                   hipMemcpyHostToDevice) == hipSuccess));
   if (!ok) {
     free_batch(b);
     set_error("q-space: table upload failed");
@@ -539,21 +833,25 @@ static void launch_L_nb(QCtx* q, QBatch& b, const double2* psi, double2* out, in
   const int C = b.C, MP = b.MP, nq = q->nq;
   const int do_anh = !(ignore_v3 && ignore_v4);
   if (do_anh) {
-    hipLaunchKernelGGL(q_unpack<NB>, dim3(MP, C), dim3(64), 0, s, psi, b.ld, b.active, b.npairs,
-                       b.pq1, b.pq2, b.offa, b.offb, MP, q->wq, q->nbose, q->valid, ignore_v3,
-                       b.alpha, b.R1);
-    int maxp = 0;
-    for (int v : b.h_np) maxp = std::max(maxp, v);
-    const size_t sm = std::max(sizeof(double2) * (size_t)q->weights_tile,
-                               sizeof(double) * (2 * (size_t)maxp * NB * NB + 2 * (size_t)nq * NB) +
-                                   sizeof(int) * 3 * (size_t)nq);
-    hipLaunchKernelGGL(q_weights<NB>, dim3(C, b.ntile), dim3(q->weights_tile), sm, s, q->X, q->Y,
-                       q->rho, q->N, nq, q->fY, q->fP, b.active, b.iqp, b.npairs, b.partner,
-                       b.apair, b.aorient, b.alpha, b.R1, MP, ignore_v4 ? 0 : 1, b.wc, b.wd,
-                       b.fpart, b.ntile);
-    const int chunk = (q->N + b.nch - 1) / b.nch;
-    hipLaunchKernelGGL(q_d2v<NB>, dim3(MP, C, b.nch), dim3(256), 0, s, q->X, q->Y, q->N, q->fY,
-                       b.active, b.npairs, b.pq1, b.pq2, MP, b.wc, b.wd, chunk, b.nch, b.dpart);
+    hipLaunchKernelGGL(q_unpack<NB>, dim3(nq, C), dim3(64), 0, s, psi, b.ld, b.active, b.pq1,
+                       b.pq2, b.offa, b.offb, MP, nq, b.apair, b.aorient, q->wq, q->nbose,
+                       q->valid, ignore_v3, b.alpha, b.R1);
+    constexpr int CB = kQWeightsChains;
+    const int ngroups = (C + CB - 1) / CB;
+    const int nblk = (b.ntile + 7) / 8 * 8 * ngroups;
+    hipLaunchKernelGGL((q_weights<NB, CB>), dim3(nblk), dim3(q->weights_tile), 0, s, q->X, q->Y,
+                       q->rho, q->N, nq, q->fY, q->fP, b.active, b.iqp, b.partner, b.alpha, b.R1,
+                       C, ngroups, b.ntile, ignore_v4 ? 0 : 1, b.wc, b.wd, b.fpart);
+    if (b.slabT) {
+      const int nblk2 = (b.nch + 7) / 8 * 8 * b.nib;
+      hipLaunchKernelGGL(q_d2v_slab<NB>, dim3(nblk2), dim3(kD2vThreads), 0, s, q->S, q->N, nq,
+                         b.active, b.items, b.blk_start, b.nib, MP, b.wc, b.wd,
+                         log2_pow2(b.slabT), b.chunk, b.nch, b.dpart);
+    } else {
+      hipLaunchKernelGGL(q_d2v<NB>, dim3(b.maxp, C, b.nch), dim3(256), 0, s, q->X, q->Y, q->N,
+                         q->fY, b.active, b.npairs, b.pq1, b.pq2, MP, b.wc, b.wd, b.chunk, b.nch,
+                         b.dpart);
+    }
   }
   hipLaunchKernelGGL(q_finish<NB>, dim3(MP + 1, C), dim3(64), 0, s, psi, b.ld, out, b.active,
                      b.iqp, b.npairs, b.pq1, b.pq2, b.offa, b.offb, MP, q->wq, q->nbose, q->valid,
@@ -896,6 +1194,13 @@ int tdscha_qctx_create(tdscha_qctx** out, int n_q, int n_bands, int n_configs, c
     return -1;
   };
   if (hipStreamCreate(&q->stream) != hipSuccess) return fail("hipStreamCreate failed");
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        ncu > 0)
+      q->n_cu = ncu;
+  }
   bool ok = hipMalloc((void**)&q->X, sizeof(double2) * ne) == hipSuccess &&
             hipMalloc((void**)&q->Y, sizeof(double2) * ne) == hipSuccess &&
             hipMalloc((void**)&q->rho, sizeof(double) * N) == hipSuccess &&
@@ -914,6 +1219,7 @@ int tdscha_qctx_create(tdscha_qctx** out, int n_q, int n_bands, int n_configs, c
        hipMemcpy(q->fY, hfy.data(), sizeof(double) * nq * nb, hipMemcpyHostToDevice) == hipSuccess &&
        hipMemcpy(q->fP, hfp.data(), sizeof(double) * nq * nb, hipMemcpyHostToDevice) == hipSuccess;
   if (!ok) return fail("upload failed");
+  if (build_slabs(q)) return fail("slab images failed");
   *out = reinterpret_cast<tdscha_qctx*>(q);
   return 0;
 }
@@ -922,7 +1228,7 @@ void tdscha_qctx_destroy(tdscha_qctx* ctx) {
   if (!ctx) return;
   QCtx* q = reinterpret_cast<QCtx*>(ctx);
   if (q->stream) (void)hipStreamSynchronize(q->stream);
-  void* p[] = {q->X, q->Y, q->rho, q->wq, q->nbose, q->valid, q->fY, q->fP};
+  void* p[] = {q->X, q->Y, q->rho, q->wq, q->nbose, q->valid, q->fY, q->fP, q->S};
   for (void* x : p)
     if (x) (void)hipFree(x);
   if (q->stream) (void)hipStreamDestroy(q->stream);
@@ -1004,7 +1310,7 @@ int tdscha_qctx_symmetrize(tdscha_qctx* ctx, int n_sym, const int* q_perm, const
   q->N = N2;
   q->neff *= n_sym;
   q->n_sym = n_sym;
-  return 0;
+  return build_slabs(q);
 }
 
 int tdscha_qspace_psi_size(int n_q, int n_bands, const int* q_pair_map, int* psi_size) {
@@ -1032,6 +1338,13 @@ int tdscha_qspace_set_tiles(tdscha_qctx* ctx, int weights_tile, int d2v_chunk, i
   q->weights_tile = weights_tile;
   q->d2v_chunk = d2v_chunk;
   q->reorth_jtile = reorth_jtile;
+  return 0;
+}
+
+int tdscha_qspace_set_d2v_form(tdscha_qctx* ctx, int form) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_qspace_set_d2v_form: NULL context");
+  TDS_REQUIRE(form == 0 || form == 1, 2, "d2v form must be 0 or 1");
+  reinterpret_cast<QCtx*>(ctx)->d2v_form = form;
   return 0;
 }
 
@@ -1217,6 +1530,18 @@ int tdscha_qspace_autotune(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
   reps = std::max(1, reps);
   const int wt_c[3] = {64, 128, 256};
   std::vector<int> ch_c = {1024, 2048, 4096, 8192};
+  if (q->S && q->d2v_form) {
+    // chunks giving 1 to 4 rounds of slab workgroups for this batch
+    QBatch b0;
+    if (build_batch(q, n_chains, iq_pert, q_pair_map, b0)) return -1;
+    const int nib = b0.nib;
+    free_batch(b0);
+    ch_c.clear();
+    for (int r = 1; r <= 4; ++r) {
+      const int c = slab_chunk(q, nib, r);
+      if (std::find(ch_c.begin(), ch_c.end(), c) == ch_c.end()) ch_c.push_back(c);
+    }
+  }
   const int saved[3] = {q->weights_tile, q->d2v_chunk, q->reorth_jtile};
   hipStream_t s = q->stream;
   hipEvent_t e0 = nullptr, e1 = nullptr;

@@ -116,6 +116,10 @@ class QSpaceEngine:
                                                       int(d2v_chunk), int(reorth_jtile)),
                    "tdscha_qspace_set_tiles")
 
+    def set_d2v_form(self, form):
+        _lib.check(_lib.lib().tdscha_qspace_set_d2v_form(self.ctx, int(form)),
+                   "tdscha_qspace_set_d2v_form")
+
     def tiles(self):
         t = np.zeros(3, np.int32)
         _lib.check(_lib.lib().tdscha_qspace_get_tiles(self.ctx, _p(t)), "tdscha_qspace_get_tiles")

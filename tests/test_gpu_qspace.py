@@ -56,6 +56,33 @@ def test_apply_L_batch_matches_oracle(L, N, iqs, flags):
     eng.close()
 
 
+@pytest.mark.parametrize("L,N", [(2, 10), (4, 157)])
+@pytest.mark.parametrize("form,chunk", [(0, 0), (0, 64), (1, 0), (1, 3), (1, 40)])
+def test_d2v_forms_and_chunks_match_oracle(L, N, form, chunk):
+    """Both forms of the d2v kernel (LDS slabs of every q-point, or one
+    workgroup per (chain, pair) block), chunks that do not divide the
+    ensemble, and a last slab that runs past its end (N = 10 with slabs of 16
+    configurations at 8 q-points, N = 157 with slabs of 2 at 64)."""
+    ens, modes, Xq, Yq, eng = _setup(L, N)
+    eng.set_d2v_form(form)
+    eng.set_tiles(64, chunk, 16)
+    iqs = [0, 3, 7] if L == 2 else [0, 1, 21, 42, 63]
+    rng = np.random.default_rng(23)
+    pms, psis, refs = [], [], []
+    for iq in iqs:
+        qs = _oracle(ens, modes, Xq, Yq, iq)
+        psi = rng.normal(size=qs.psi_size) + 1j * rng.normal(size=qs.psi_size)
+        pms.append(qs.q_pair_map)
+        psis.append(psi)
+        refs.append(qs.apply_full_L(psi))
+    outs = eng.apply_L(iqs, pms, psis)
+    for iq, o, r in zip(iqs, outs, refs):
+        err = _rel(o, r)
+        print("L=%d N=%d form=%d chunk=%d iq=%d rel err %.2e" % (L, N, form, chunk, iq, err))
+        assert err < 1e-12
+    eng.close()
+
+
 def test_lanczos_matches_oracle_chains():
     ens, modes, Xq, Yq, eng = _setup(4, 150)
     iqs = [0, 5, 42]
