@@ -113,12 +113,15 @@ def cpu_baseline(ens, lanc, n_sample_cfg, budget_s):
                                                     "oracle restatement", dt)}
 
 
-def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True):
+def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True, tiles=None,
+               n_apply=0):
     """Batched uneven q-space chains on the same ensemble: one chain per
     q-point of the 4x4x4 grid, each from a seeded random unit complex
     perturbation of its 6 bands, stopping at |b| < tol or at a per-q cap.
-    Timed as one whole-run hipGraph launch (autotuned tiles) and as per-chain
-    plain launches; L-applications/s counts every a coefficient computed."""
+    Timed as one whole-run hipGraph launch (autotuned tiles, or `tiles`) and
+    as per-chain plain launches; L-applications/s counts every a coefficient
+    computed.  With n_apply > 0 it only runs that many full-batch L
+    applications (for profiles)."""
     from tdscha_amd.qlanczos import QSpaceEngine
     from tdscha_amd.qspace import bloch_ensemble, pair_map
     t = time.time()
@@ -137,7 +140,19 @@ def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True):
         psi[:nb] = v / np.linalg.norm(v)
         pms.append(pm)
         psis.append(psi)
-    tiles, tune_ms = eng.autotune(iqs, pms, reps=2)
+    if tiles is None:
+        tiles, tune_ms = eng.autotune(iqs, pms, reps=2)
+    else:
+        eng.set_tiles(*tiles)
+        tune_ms = None
+    if n_apply:
+        # profile mode: full-batch L applications only (kernel times come
+        # from the profiler; each call here also builds its batch tables)
+        for _ in range(n_apply):
+            eng.apply_L(iqs, pms, psis)
+        eng.close()
+        return {"workload": "q-space full-batch L applications (64 chains)", "n_apply": n_apply,
+                "tiles": list(tiles)}
     n_steps = int(caps.max())
     t = time.perf_counter()
     chains, dev_ms = eng.lanczos(iqs, pms, psis, n_steps, caps=caps, tol=tol, use_graph=True)
@@ -148,7 +163,8 @@ def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True):
                        "tol %g, FP64 complex" % (ens.N, cap_range[0], cap_range[1], tol),
            "l_applications": int(n_l), "batched_graph_value": n_l / wall,
            "batched_graph_wall_s": wall, "batched_graph_device_ms": dev_ms,
-           "unit": "L-applications/s", "tiles": list(tiles), "autotune_best_L_ms": tune_ms,
+           "unit": "L-applications/s", "tiles": list(tiles), "d2v_form": eng.d2v_form(),
+           "autotune_best_L_ms": tune_ms,
            "setup_s": t_setup, "chains_stopped_on_tol": int(sum(len(b) < len(a)
                                                                for a, b, _ in chains))}
     if per_chain:
@@ -169,7 +185,7 @@ def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True):
         # differences grow along long recurrences).  With equal tiles the runs are
         # bit-identical (tests/test_gpu_qspace.py).
         out.update({"per_chain_value": n_l / wall1, "per_chain_wall_s": wall1,
-                    "per_chain_tiles": list(tiles1),
+                    "per_chain_tiles": list(tiles1), "per_chain_d2v_form": eng.d2v_form(),
                     "per_chain_vs_batched_max_rel_diff_a_first50": worst50,
                     "per_chain_vs_batched_max_rel_diff_a": worst,
                     "batched_speedup": wall1 / wall})
@@ -191,6 +207,12 @@ def main():
     ap.add_argument("--no-qspace", action="store_true")
     ap.add_argument("--qspace-only", action="store_true",
                     help="run only the batched q-space chains and print their JSON")
+    ap.add_argument("--qspace-tiles", default=None,
+                    help="q-space tiles W,C,R instead of the autotuner's")
+    ap.add_argument("--qspace-apply", type=int, default=0,
+                    help="run only this many full-batch q-space L applications (profiles)")
+    ap.add_argument("--no-per-chain", action="store_true",
+                    help="skip the per-chain q-space comparison run")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="bytes of one device-to-device copy to run first (counter scaling)")
     args = ap.parse_args()
@@ -207,7 +229,9 @@ def main():
 
     ens = build_workload(args.supercell, args.configs, 300.0, seed=0)
     if args.qspace_only:
-        print(json.dumps(qspace_run(ens)), flush=True)
+        qt = tuple(int(x) for x in args.qspace_tiles.split(",")) if args.qspace_tiles else None
+        print(json.dumps(qspace_run(ens, per_chain=not args.no_per_chain, tiles=qt,
+                                    n_apply=args.qspace_apply)), flush=True)
         return
     lanc = Lanczos(ens)
     lanc.gamma_only = True

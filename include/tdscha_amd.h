@@ -212,7 +212,7 @@ int tdscha_qspace_lanczos_run(tdscha_qctx* ctx, int n_chains, const int* iq_pert
  * weights kernel (64, 128, 256), configurations per workgroup of the d2v
  * kernel (0 = automatic) and basis vectors per workgroup of the
  * reorthogonalisation (8, 16, 32).  tdscha_qspace_autotune times one L
- * application of the given batch for every weights/d2v candidate (reps
+ * application of the given batch for every weights/d2v-chunk/d2v-form candidate (reps
  * repetitions each), keeps the fastest and returns it in tiles_out[3].
  */
 int tdscha_qspace_set_tiles(tdscha_qctx* ctx, int weights_tile, int d2v_chunk, int reorth_jtile);
@@ -222,9 +222,11 @@ int tdscha_qspace_get_tiles(const tdscha_qctx* ctx, int* tiles_out);
  * q-point in LDS and gives each thread one (chain, pair) block; 0 gives each
  * workgroup one (chain, pair) block and each thread a configuration.  Grids
  * too large for a one-configuration slab use form 0 whatever is set.  Both
- * compute the same sums in different orders.
+ * compute the same sums in different orders.  tdscha_qspace_autotune also
+ * picks the form; get returns the form in use.
  */
 int tdscha_qspace_set_d2v_form(tdscha_qctx* ctx, int form);
+int tdscha_qspace_get_d2v_form(const tdscha_qctx* ctx, int* form);
 int tdscha_qspace_autotune(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
                            const int* q_pair_map, int reps, int* tiles_out, double* best_ms);
 

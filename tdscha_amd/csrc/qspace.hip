@@ -29,6 +29,7 @@
 // n_sym N_eff (JQ:621), and every kernel above runs unchanged.
 // All sums are fixed-order trees or loops: results do not depend on timing.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <vector>
 
@@ -1341,6 +1342,13 @@ int tdscha_qspace_set_tiles(tdscha_qctx* ctx, int weights_tile, int d2v_chunk, i
   return 0;
 }
 
+int tdscha_qspace_get_d2v_form(const tdscha_qctx* ctx, int* form) {
+  TDS_REQUIRE(ctx && form, 2, "tdscha_qspace_get_d2v_form: NULL argument");
+  const QCtx* q = reinterpret_cast<const QCtx*>(ctx);
+  *form = q->S && q->d2v_form ? 1 : 0;
+  return 0;
+}
+
 int tdscha_qspace_set_d2v_form(tdscha_qctx* ctx, int form) {
   TDS_REQUIRE(ctx != nullptr, 2, "tdscha_qspace_set_d2v_form: NULL context");
   TDS_REQUIRE(form == 0 || form == 1, 2, "d2v form must be 0 or 1");
@@ -1528,69 +1536,80 @@ int tdscha_qspace_autotune(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
   TDS_REQUIRE(n_chains >= 1 && n_chains <= kQMaxChains, 2, "n_chains must be in [1, 256]");
   QCtx* q = reinterpret_cast<QCtx*>(ctx);
   reps = std::max(1, reps);
-  const int wt_c[3] = {64, 128, 256};
-  std::vector<int> ch_c = {1024, 2048, 4096, 8192};
-  if (q->S && q->d2v_form) {
-    // chunks giving 1 to 4 rounds of slab workgroups for this batch
+  // candidates (d2v form, weights tile, d2v chunk): the slab form with the
+  // chunks that give 1 to 4 rounds of its workgroups for this batch, and the
+  // per-block form with fixed chunks
+  const int saved[4] = {q->weights_tile, q->d2v_chunk, q->reorth_jtile, q->d2v_form};
+  std::vector<std::array<int, 3>> cand;
+  for (int wt : {64, 128, 256})
+    for (int chk : {1024, 2048, 4096, 8192})
+      if (chk <= q->N || chk == 1024) cand.push_back({0, wt, std::min(chk, q->N)});
+  if (q->S) {
+    q->d2v_form = 1;
     QBatch b0;
-    if (build_batch(q, n_chains, iq_pert, q_pair_map, b0)) return -1;
+    if (build_batch(q, n_chains, iq_pert, q_pair_map, b0)) {
+      q->d2v_form = saved[3];
+      return -1;
+    }
     const int nib = b0.nib;
     free_batch(b0);
-    ch_c.clear();
+    std::vector<int> ch;
     for (int r = 1; r <= 4; ++r) {
       const int c = slab_chunk(q, nib, r);
-      if (std::find(ch_c.begin(), ch_c.end(), c) == ch_c.end()) ch_c.push_back(c);
+      if (std::find(ch.begin(), ch.end(), c) == ch.end()) ch.push_back(c);
     }
+    for (int wt : {64, 128, 256})
+      for (int c : ch) cand.push_back({1, wt, c});
   }
-  const int saved[3] = {q->weights_tile, q->d2v_chunk, q->reorth_jtile};
   hipStream_t s = q->stream;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    q->d2v_form = saved[3];
     set_error("tdscha_qspace_autotune: event creation failed");
     return -1;
   }
   double best = 1e300;
-  int bw = saved[0], bc = saved[1];
+  int bw = saved[0], bc = saved[1], bf = saved[3];
   double2 *dpsi = nullptr, *dout = nullptr;
   int rc = 0;
-  for (int wt : wt_c)
-    for (int chk : ch_c) {
-      if (chk > q->N && chk != ch_c[0]) continue;
-      q->weights_tile = wt;
-      q->d2v_chunk = std::min(chk, q->N);
-      QBatch b;
-      rc = build_batch(q, n_chains, iq_pert, q_pair_map, b);
-      if (rc) break;
-      const size_t nbytes = sizeof(double2) * (size_t)n_chains * b.ld;
-      if (!dpsi) {
-        if (hipMalloc((void**)&dpsi, nbytes * 2) != hipSuccess ||
-            hipMemset(dpsi, 0, nbytes * 2) != hipSuccess) {
-          free_batch(b);
-          set_error("tdscha_qspace_autotune: allocation failed");
-          rc = -1;
-          break;
-        }
-        dout = dpsi + (size_t)n_chains * b.ld;
-      }
-      launch_L(q, b, dpsi, dout, 0, 0, 0, s);   // warm-up
-      (void)hipEventRecord(e0, s);
-      for (int r = 0; r < reps; ++r) launch_L(q, b, dpsi, dout, 0, 0, 0, s);
-      (void)hipEventRecord(e1, s);
-      if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) {
+  for (const auto& cd : cand) {
+    q->d2v_form = cd[0];
+    q->weights_tile = cd[1];
+    q->d2v_chunk = cd[2];
+    QBatch b;
+    rc = build_batch(q, n_chains, iq_pert, q_pair_map, b);
+    if (rc) break;
+    const size_t nbytes = sizeof(double2) * (size_t)n_chains * b.ld;
+    if (!dpsi) {
+      if (hipMalloc((void**)&dpsi, nbytes * 2) != hipSuccess ||
+          hipMemset(dpsi, 0, nbytes * 2) != hipSuccess) {
         free_batch(b);
-        set_error("tdscha_qspace_autotune: timing run failed");
+        set_error("tdscha_qspace_autotune: allocation failed");
         rc = -1;
         break;
       }
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms / reps < best) {
-        best = ms / reps;
-        bw = wt;
-        bc = q->d2v_chunk;
-      }
-      free_batch(b);
+      dout = dpsi + (size_t)n_chains * b.ld;
     }
+    launch_L(q, b, dpsi, dout, 0, 0, 0, s);   // warm-up
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < reps; ++r) launch_L(q, b, dpsi, dout, 0, 0, 0, s);
+    (void)hipEventRecord(e1, s);
+    if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess) {
+      free_batch(b);
+      set_error("tdscha_qspace_autotune: timing run failed");
+      rc = -1;
+      break;
+    }
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms / reps < best) {
+      best = ms / reps;
+      bw = cd[1];
+      bc = cd[2];
+      bf = cd[0];
+    }
+    free_batch(b);
+  }
   if (dpsi) (void)hipFree(dpsi);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1598,10 +1617,12 @@ int tdscha_qspace_autotune(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
     q->weights_tile = saved[0];
     q->d2v_chunk = saved[1];
     q->reorth_jtile = saved[2];
+    q->d2v_form = saved[3];
     return rc;
   }
   q->weights_tile = bw;
   q->d2v_chunk = bc;
+  q->d2v_form = bf;
   if (tiles_out) {
     tiles_out[0] = q->weights_tile;
     tiles_out[1] = q->d2v_chunk;

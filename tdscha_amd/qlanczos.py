@@ -120,6 +120,12 @@ class QSpaceEngine:
         _lib.check(_lib.lib().tdscha_qspace_set_d2v_form(self.ctx, int(form)),
                    "tdscha_qspace_set_d2v_form")
 
+    def d2v_form(self):
+        f = np.zeros(1, np.int32)
+        _lib.check(_lib.lib().tdscha_qspace_get_d2v_form(self.ctx, _p(f)),
+                   "tdscha_qspace_get_d2v_form")
+        return int(f[0])
+
     def tiles(self):
         t = np.zeros(3, np.int32)
         _lib.check(_lib.lib().tdscha_qspace_get_tiles(self.ctx, _p(t)), "tdscha_qspace_get_tiles")
