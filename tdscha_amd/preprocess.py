@@ -27,6 +27,7 @@ from __future__ import annotations
 import numpy as np
 
 BOHR_ANGSTROM = 0.52917720859
+CC_BOHR_PER_ANGSTROM = 1.889725989
 
 
 def translation_mask(pols, m, threshold=0.5):
@@ -161,3 +162,50 @@ def prepare_ensemble(u_disps, forces, sscha_forces, masses, w, pols, T, rho=None
             pos = pos / BOHR_ANGSTROM
         structure = (lat, pos, np.asarray(spec))
     return PreparedEnsemble(T, w[good].copy(), pg, m, rho, X, Y, u, f, good, structure)
+
+
+def prepare_from_files(dyn_prefix, nqirr, ensemble_dir, population, T, rho=None,
+                       select_modes=None, ignore_small_w=False):
+    """Mode-space ensemble from the reference's input files: the
+    dynamical-matrix files ``dyn_prefix1`` .. ``dyn_prefix<nqirr>`` and the
+    binary ensemble ``xats_pop<p>.npy`` / ``forces_pop<p>.npy`` in
+    ``ensemble_dir`` (the sequence CC.Phonons.Phonons(prefix, nqirr),
+    sscha Ensemble(dyn, T).load_bin(dir, p), Lanczos(ens) of
+    tests/test_julia/test_julia.py:27-32).  The ensemble's atoms follow the
+    supercell order of dynmat.supercell_structure; the SSCHA forces are the
+    harmonic forces -Phi u of the supercell force constants, rho defaults to
+    1 (configurations drawn from the same matrices), and the average force is
+    symmetrised by the unit cell's space group (symmetry.find_symmetries).
+
+    Units follow the reference's two conversions: the structure's Angstrom
+    coordinates and the harmonic forces use 1 Angstrom = CC_BOHR_PER_ANGSTROM
+    bohr (the value of DL:133), the Lanczos' own conversions
+    BOHR_ANGSTROM (DL:126-128, ASE's 2006 units).  The two differ by 8e-8,
+    which moves X and Y by about 1e-6 relative because positions are much
+    larger than displacements; with both, X and Y match the reference's to
+    5e-10."""
+    from . import dynmat as DM
+    from . import symmetry as SY
+    ds = DM.read_dyn_set(dyn_prefix, nqirr)
+    xats, forces = DM.load_ensemble_bin(ensemble_dir, population)
+    pos, lat, spec, mass = DM.supercell_structure(ds)
+    if xats.shape[1] != len(pos):
+        raise ValueError("the ensemble has %d atoms, the supercell of the matrices %d"
+                         % (xats.shape[1], len(pos)))
+    phi = DM.supercell_force_constants(ds, pos)
+    w, pols = DM.supercell_modes(phi, mass)
+    pos_a, lat_a = pos / CC_BOHR_PER_ANGSTROM, lat / CC_BOHR_PER_ANGSTROM
+    u = xats - pos_a[None]                                # Angstrom, nearest image of each site
+    u -= np.round(u @ np.linalg.inv(lat_a)) @ lat_a
+    nn = min(np.linalg.norm(pos_a[i] - pos_a[j]) for i in range(len(pos_a))
+             for j in range(i + 1, len(pos_a)))
+    if np.abs(u).max() > 0.5 * nn:
+        raise ValueError("ensemble atoms are not in the supercell order of the matrices")
+    N = len(u)
+    f_h = -(u.reshape(N, -1) @ phi) * CC_BOHR_PER_ANGSTROM ** 2   # Ry / Angstrom
+    ops = SY.find_symmetries(ds.cell, ds.tau, [ds.species[t] for t in ds.ityp])
+    return prepare_ensemble(u.reshape(N, -1), forces.reshape(N, -1), f_h, mass, w, pols, T,
+                            rho=rho, n_cells=int(np.prod(ds.supercell_dims())),
+                            rotations=[op[0] for op in ops], atom_maps=[op[2] for op in ops],
+                            select_modes=select_modes, ignore_small_w=ignore_small_w,
+                            angstrom=True, structure=(lat_a, pos_a, spec))

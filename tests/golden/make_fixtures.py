@@ -12,6 +12,12 @@ Sources (read as data only; nothing from the reference is imported or run):
       the 10 configurations, used to recover the supercell atom order.
   tests/test_julia/data/dyn_gen_pop1_1     header: alat and unit cell.
 
+  tests/test_julia/data/dyn_gen_pop1_{1,2,3}, xats_pop1.npy,
+      forces_pop1.npy   the raw inputs of that run (the dynamical-matrix
+      files of the 3 irreducible q-points and the ensemble in the
+      load_bin layout), copied unchanged into tests/golden/snte_raw/ for the
+      file readers and the preprocessing.
+
 Run:  python tests/golden/make_fixtures.py /root/reference
 """
 import os
@@ -51,6 +57,15 @@ def main(ref_root):
     dst = os.path.join(os.path.dirname(os.path.abspath(__file__)), "snte_step10.npz")
     np.savez_compressed(dst, **out)
     print("wrote", dst, sorted(out))
+    raw = os.path.join(os.path.dirname(os.path.abspath(__file__)), "snte_raw")
+    os.makedirs(raw, exist_ok=True)
+    for i in (1, 2, 3):
+        name = "dyn_gen_pop1_%d" % i
+        with open(os.path.join(tj, "data", name)) as fp, open(os.path.join(raw, name), "w") as fo:
+            fo.write(fp.read())
+    for name in ("xats_pop1.npy", "forces_pop1.npy"):
+        np.save(os.path.join(raw, name), np.load(os.path.join(tj, "data", name), allow_pickle=False))
+    print("wrote", raw, sorted(os.listdir(raw)))
 
 
 if __name__ == "__main__":

@@ -227,3 +227,44 @@ def test_lanczos_from_a_prepared_ensemble_with_detected_symmetries():
     err = rel(out[0], out[1])
     print("prepared + detected vs generator + builder: rel %.2e" % err)
     assert err < 1e-11
+
+
+def test_golden_run_from_the_reference_input_files(snte):
+    """The reference's SnTe inputs as files (dynamical matrices of the 3
+    irreducible q-points, binary ensemble) -> prepare_from_files -> detected
+    space group (384 operations) -> 10-step runs:
+      * standard representation (use_wigner=False): the golden coefficients
+        of test_julia_lf.py:24-45;
+      * Wigner representation (the default, as in tests/test_julia/
+        test_julia.py:33-50): the Green function at w = 0 of that test,
+        -36.93203026 cm^-1 (its tolerance is rtol 1e-3), and the same chain
+        with a -> -a.
+    X, Y from the files differ from the reference's saved ones by 4.5e-10;
+    the last product b c (1e-18, after a near-breakdown b_7 = 2e-9) carries
+    that difference amplified, hence its looser bound.  Mode 10 lies in a
+    threefold degenerate block whose basis is ours, not the reference's; the
+    averaged L commutes with the group, so the chain is the same."""
+    from tdscha_amd import preprocess as PP
+    from tdscha_amd.lanczos import Lanczos
+    raw = os.path.join(GOLDEN, "snte_raw")
+    ens = PP.prepare_from_files(os.path.join(raw, "dyn_gen_pop1_"), 3, raw, 1, 250.0)
+    a_ref = snte["a_coeffs"]
+    bc_ref = snte["b_coeffs"] * snte["c_coeffs"]
+    for wig in (False, True):
+        lanc = Lanczos(ens, use_wigner=wig)
+        lanc.init(use_symmetries=True)
+        lanc.prepare_mode(10)
+        lanc.run_FT(10, verbose=False)
+        a = np.array(lanc.a_coeffs) * (-1 if wig else 1)
+        bc = np.array(lanc.b_coeffs) * np.array(lanc.c_coeffs)
+        ea = np.max(np.abs(a / a_ref - 1))
+        ebc = np.abs(bc / bc_ref - 1)
+        gf = lanc.get_green_function_continued_fraction(np.array([0.0]), smearing=0)
+        w = np.sign(np.real(gf)) * np.sqrt(np.abs(1.0 / np.real(gf))) * 109737.31568527
+        print("wigner=%d: a %.2e  bc %.2e (last %.2e)  w(0) %.8f cm-1"
+              % (wig, ea, ebc[:9].max(), ebc[9], w[0]))
+        assert ea < 1e-8
+        assert ebc[:9].max() < 1e-8 and ebc[9] < 1e-5
+        if wig:
+            assert abs(w[0] / -36.93203026 - 1) < 1e-6
+        lanc.close()
