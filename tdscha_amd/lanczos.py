@@ -7,7 +7,8 @@ ensemble (DL:183-452), ``init`` / ``prepare_symmetrization`` (DL:564,
 (DL:2023-2091, 1705-1749), ``apply_full_L`` (DL:3549-3651), ``run_FT``
 (DL:5958-6491), ``get_green_function_continued_fraction`` (DL:5032-5162),
 ``mask_dot_wigner`` (DL:5905-5954), ``save_status`` / ``load_status`` /
-``save_abc`` / ``load_abc`` (DL:3653-3839).
+``save_abc`` / ``load_abc`` (DL:3653-3839) and the standalone C++ driver's
+files ``prepare_input_files`` / ``load_from_input_files`` (DL:873-1044).
 
 Every L application, the perturbed averages, the gamma-only translational
 projection, the batched device-resident Lanczos and the continued fraction
@@ -22,7 +23,9 @@ from __future__ import annotations
 
 import ctypes as C
 import difflib
+import json
 import os
+import warnings
 
 import numpy as np
 
@@ -533,6 +536,82 @@ class Lanczos(object):
             self.perturbation_modulus = float(data["perturbation_modulus"])
         self.n_syms = int(data.get("n_syms", 1))
         self.initialized = False
+
+    def prepare_input_files(self, root_name="tdscha", n_steps=100, start_from_scratch=True,
+                            directory=".", run_symm=False):
+        """DL:873-978: the input files of the reference's standalone C++
+        driver (``<root>.json``, ``.ndegs``, ``.blockid``, ``.masses``,
+        ``.freqs``, ``.rho``, ``.X.dat``, ``.Y.dat``, ``.psi``, and per
+        symmetry block ``.block<i>`` and ``.symsb<i>``), in the same layout,
+        so that a run prepared here can go to that driver and back."""
+        if not start_from_scratch:
+            raise NotImplementedError("Error, restarting from a previous calculation is not yet "
+                                      "implemented.")
+        if self.symmetries is None:
+            raise ValueError("call init() before prepare_input_files")
+        nsyms = int(np.shape(self.symmetries[0])[0])
+        json_data = {"T": float(self.T), "n_steps": int(n_steps),
+                     "ignore_v2": bool(self.ignore_harmonic), "ignore_v3": bool(self.ignore_v3),
+                     "ignore_v4": bool(self.ignore_v4), "use_wigner": bool(self.use_wigner),
+                     "run_sym": bool(run_symm),
+                     "data": {"n_configs": int(self.N), "n_modes": int(self.n_modes),
+                              "n_syms": nsyms, "n_blocks": len(self.symmetries),
+                              "perturbation_modulus": float(self.perturbation_modulus),
+                              "reverse": bool(self.reverse_L), "shift": float(self.shift_value)}}
+        os.makedirs(directory, exist_ok=True)
+        root = os.path.join(directory, root_name)
+        with open(root + ".json", "w") as fp:
+            json.dump(json_data, fp)
+        np.savetxt(root + ".ndegs", self.N_degeneracy, fmt="%d")
+        np.savetxt(root + ".blockid", self.sym_block_id, fmt="%d")
+        np.savetxt(root + ".masses", self.m)
+        np.savetxt(root + ".freqs", self.w)
+        np.savetxt(root + ".rho", self.rho)
+        np.savetxt(root + ".X.dat", self.X)
+        np.savetxt(root + ".Y.dat", self.Y)
+        np.savetxt(root + ".psi", self.psi)
+        for i, (blk, sym) in enumerate(zip(self.degenerate_space, self.symmetries)):
+            np.savetxt(root + ".block%d" % i, blk, fmt="%d")
+            ns, b1, b2 = sym.shape
+            with open(root + ".symsb%d" % i, "w") as fp:
+                for isym in range(ns):
+                    for k1 in range(b1):
+                        fp.write("".join(" {:22.16f}".format(sym[isym, k1, k2]) for k2 in range(b2)))
+                        fp.write("\n")
+                    fp.write("\n")
+
+    def load_from_input_files(self, root_name="tdscha", directory="."):
+        """DL:981-1044: the coefficients (``<root>.abc``) and, when present,
+        the Krylov bases and norms a run of the C++ driver wrote, with the
+        run's settings from ``<root>.json``."""
+        if not os.path.exists(directory):
+            raise IOError("Error, the specified directory '%s' does not exist" % directory)
+        abc_file = os.path.join(directory, "%s.abc" % root_name)
+        if not os.path.exists(abc_file):
+            raise IOError("Error, the file '%s' does not exist." % abc_file)
+        abc = np.atleast_2d(np.loadtxt(abc_file, dtype=np.double))
+        self.a_coeffs = abc[:, 0]
+        self.b_coeffs = abc[:, 1]
+        self.c_coeffs = abc[:, 2]
+        for attr, suffix in (("basis_Q", "qbasis.out"), ("basis_P", "pbasis.out"),
+                             ("s_norm", "snorm.out")):
+            fname = os.path.join(directory, "%s.%s" % (root_name, suffix))
+            if os.path.exists(fname):
+                v = np.loadtxt(fname, dtype=np.double)
+                setattr(self, attr, v.ravel() if attr == "s_norm" else v)
+            else:
+                warnings.warn("File %s not found. %s not loaded." % (fname, attr))
+        with open(os.path.join(directory, root_name + ".json")) as fp:
+            js = json.load(fp)
+        self.perturbation_modulus = js["data"]["perturbation_modulus"]
+        self.T = js["T"]
+        self.ignore_harmonic = js["ignore_v2"]
+        self.ignore_v3 = js["ignore_v3"]
+        self.ignore_v4 = js["ignore_v4"]
+        self.N = js["data"]["n_configs"]
+        self.n_modes = js["data"]["n_modes"]
+        self.reverse_L = js["data"]["reverse"]
+        self.shift_value = js["data"]["shift"]
 
     def close(self):
         if self.engine is not None:

@@ -18,6 +18,11 @@ Sources (read as data only; nothing from the reference is imported or run):
       load_bin layout), copied unchanged into tests/golden/snte_raw/ for the
       file readers and the preprocessing.
 
+  tests/test_julia/test_c_code_lanczos/mode10.{json,freqs,masses,rho,ndegs,
+      blockid,block0-8,psi}   the input files the reference's
+      prepare_input_files wrote for the C++ driver on the same system
+      (mode 10, 5 steps), copied unchanged into tests/golden/snte_c_input/.
+
 Run:  python tests/golden/make_fixtures.py /root/reference
 """
 import os
@@ -66,6 +71,14 @@ def main(ref_root):
     for name in ("xats_pop1.npy", "forces_pop1.npy"):
         np.save(os.path.join(raw, name), np.load(os.path.join(tj, "data", name), allow_pickle=False))
     print("wrote", raw, sorted(os.listdir(raw)))
+    cin = os.path.join(os.path.dirname(os.path.abspath(__file__)), "snte_c_input")
+    os.makedirs(cin, exist_ok=True)
+    names = ["mode10.%s" % e for e in ("json", "freqs", "masses", "rho", "ndegs", "blockid", "psi")]
+    names += ["mode10.block%d" % i for i in range(len(out["ref_ndegs"]))]
+    for name in names:
+        with open(os.path.join(cdir, name)) as fp, open(os.path.join(cin, name), "w") as fo:
+            fo.write(fp.read())
+    print("wrote", cin, sorted(os.listdir(cin)))
 
 
 if __name__ == "__main__":
