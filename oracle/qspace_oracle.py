@@ -16,6 +16,8 @@ Numpy restatement, complex128, of the reference's q-space path:
   * point-group images: every configuration rotated by the q-space symmetry
     matrices, x_rot = S_j x (JQ:496-502, matrices built as in QL:1550-1633),
     normalised by n_syms * N_eff (JQ:621).
+  * the Chebyshev moments of the kernel polynomial method
+    (Modules/QSpaceKPM.py:260-454), one moment per L application.
 Only the tests use this file.
 
 Pins (tests/test_qspace_oracle.py): on the 2x2x2 grid (the reference's own
@@ -32,7 +34,9 @@ only through the shared code path and the invariants).  The symmetry
 matrices are pinned directly: the Bloch components of every configuration
 rotated in real space equal the rotated Bloch components (1e-13, all 48
 operations, 2x2x2 and 3x3x3 grids); with them, the 2x2x2 q-space chains equal
-the real-space chains averaged over point group x translations.
+the real-space chains averaged over point group x translations.  The KPM
+moments are pinned by the harmonic case, where L is diagonal in the band
+basis and the moments are T_n of the known eigenvalues (closed form).
 """
 from __future__ import annotations
 
@@ -304,3 +308,22 @@ def hermitian_lanczos(matvec, psi0, mask, n_iter, tol=EPS, mgs=True):
         b.append(bk)
         c.append(ck)
     return np.array(a), np.array(b), np.array(c), np.array(Q)
+
+
+def kpm_moments_direct(matvec, psi0, mask, n_moments, rescale_a, rescale_b):
+    """Chebyshev moments mu_n = <v0, T_n(L~) v0>_M, L~ = (L - b) / a, v0 the
+    metric-normalised psi0, each from the three-term recurrence
+    T_{n+1} = 2 L~ T_n - T_{n-1} carried to n (one L application per moment,
+    no moment-doubling): the quantity QSpaceKPM.run_KPM computes
+    (Modules/QSpaceKPM.py:260-454) two at a time.  Returns (moments, norm)."""
+    dot = lambda u, v: float(np.real(np.conj(u).dot(v * mask)))
+    norm = np.sqrt(dot(psi0, psi0))
+    v0 = psi0 / norm
+    Lt = lambda v: (matvec(v) - rescale_b * v) / rescale_a
+    mu = np.zeros(n_moments)
+    tm, t = None, v0
+    for n in range(n_moments):
+        mu[n] = dot(v0, t)
+        tn = Lt(t) if n == 0 else 2 * Lt(t) - tm
+        tm, t = t, tn
+    return mu, norm

@@ -179,3 +179,40 @@ def test_chain_with_point_group_equals_real_space_space_group(iq, band):
     print("a q-space", aq[:3], "real", ar[:3])
     np.testing.assert_allclose(aq, ar, rtol=1e-9)
     np.testing.assert_allclose(bq, br, rtol=1e-9)
+
+
+def test_kpm_moments_harmonic_closed_form_and_doubling_identity():
+    """Oracle KPM moments (kpm_moments_direct).  Harmonic L on a band of the
+    R sector is -w^2: the moments are T_n(x) = cos(n arccos x) of its rescaled
+    eigenvalue.  With the anharmonic terms, L is self-adjoint under the Wigner
+    metric, so mu_{m+n} + mu_{|m-n|} = 2 <T_m v, T_n v>_M (the identity the
+    device run uses to get two moments per L application)."""
+    ens, modes, Xq, Yq = _system(L=2, N=30)
+    w = modes.w_q[:, 0]
+    band = int(np.argmax(w))
+    qs = _qsys(modes, Xq, Yq, ens, 0, ignore_v3=True, ignore_v4=True)
+    a, b = 2.0 * w.max() ** 2, -1.5 * w.max() ** 2
+    psi = np.zeros(qs.psi_size, complex)
+    psi[band] = 1.0
+    mu, norm = QO.kpm_moments_direct(qs.apply_full_L, psi, qs.mask(), 20, a, b)
+    x = (-w[band] ** 2 - b) / a
+    assert abs(norm - 1) < 1e-15
+    assert np.abs(mu - np.cos(np.arange(20) * np.arccos(x))).max() < 1e-12
+    # anharmonic: the doubling identity from explicit Chebyshev vectors
+    qs = _qsys(modes, Xq, Yq, ens, 0)
+    # R-sector start: the Chebyshev vectors then keep Hermitian self-pair
+    # blocks, the space on which L is self-adjoint under the metric
+    psi = np.zeros(qs.psi_size, complex)
+    psi[:qs.nb] = np.random.default_rng(4).normal(size=qs.nb)
+    a, b = 4.0 * w.max() ** 2, -2.0 * w.max() ** 2
+    mask = qs.mask()
+    dot = lambda u, v: float(np.real(np.conj(u).dot(v * mask)))
+    mu, norm = QO.kpm_moments_direct(qs.apply_full_L, psi, mask, 12, a, b)
+    Lt = lambda v: (qs.apply_full_L(v) - b * v) / a
+    T = [psi / norm]
+    T.append(Lt(T[0]))
+    for n in range(2, 7):
+        T.append(2 * Lt(T[-1]) - T[-2])
+    for m in range(6):
+        for n in range(m + 1):
+            assert abs(mu[m + n] + mu[m - n] - 2 * dot(T[m], T[n])) < 1e-10 * max(1, abs(mu[m + n]))
