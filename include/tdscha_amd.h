@@ -208,6 +208,23 @@ int tdscha_qspace_lanczos_run(tdscha_qctx* ctx, int n_chains, const int* iq_pert
                               double* c_out, int* n_done, double* elapsed_ms);
 
 /*
+ * Kernel polynomial method for a batch of chains (replaces the loop of
+ * QSpaceKPM.run_KPM, Modules/QSpaceKPM.py:260-454): Chebyshev moments
+ * mu_n = <v0, T_n(L~) v0>_M of L~ = (L - rescale[2c+1]) / rescale[2c], two
+ * per L application, n = 0 .. n_moments-1, every step on the device (one
+ * graph launch with use_graph).  vm, v (complex, concatenated over chains in
+ * each chain's psi layout) hold T_{n-1}, T_n: for a fresh run (n_have = 0)
+ * both hold the metric-normalised v0 and moments[c][0] is set to 1; to
+ * continue, n_have moments of moments[c][0 .. n_have) and the vm, v of that
+ * run.  On return moments[c][0 .. n_moments) and the final vm, v.
+ */
+int tdscha_qspace_kpm_run(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                          const int* q_pair_map, const double* rescale, int n_moments,
+                          int n_have, double* moments, double* vm, double* v,
+                          int ignore_harmonic, int ignore_v3, int ignore_v4, int use_graph,
+                          double* elapsed_ms);
+
+/*
  * Tile choices of the q-space kernels: configurations per workgroup of the
  * weights kernel (64, 128, 256), configurations per workgroup of the d2v
  * kernel (0 = automatic) and basis vectors per workgroup of the

@@ -65,6 +65,9 @@ def _signatures(L):
                                             _vp, C.POINTER(C.c_double)]
     L.tdscha_qspace_set_tiles.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.tdscha_qspace_set_d2v_form.argtypes = [_vp, C.c_int]
+    L.tdscha_qspace_kpm_run.argtypes = [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp,
+                                        _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        _vp]
     L.tdscha_qspace_get_d2v_form.argtypes = [_vp, _vp]
     L.tdscha_qspace_get_tiles.argtypes = [_vp, _vp]
     L.tdscha_qspace_autotune.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, _vp,

@@ -1135,6 +1135,49 @@ static void host_mask(const QBatch& b, int nq, int nb, const int* pair_map, int 
     }
 }
 
+// ------------------------------------------------- kernel polynomial method
+// One work-group per chain: d = <x, y>_M (fixed-order sum), then moment m of
+// chain c is d (mode 0) or 2 d - mom[c][ref] (mode 1: mu_{2n} with ref 0,
+// mu_{2n+1} with ref 1; QSK:346-390).
+__global__ void __launch_bounds__(256) kpm_moment(const double2* __restrict__ x,
+                                                  const double2* __restrict__ y,
+                                                  const double* __restrict__ mask, int ld,
+                                                  double* __restrict__ mom, int n_mom, int m,
+                                                  int mode, int ref) {
+  const int c = blockIdx.x, t = threadIdx.x;
+  const double2* xc = x + (long)c * ld;
+  const double2* yc = y + (long)c * ld;
+  const double* mc = mask + (long)c * ld;
+  double s = 0.0;
+  for (int e = t; e < ld; e += blockDim.x) s += mc[e] * (xc[e].x * yc[e].x + xc[e].y * yc[e].y);
+  __shared__ double red[4];
+  s = wave_sum(s);
+  if ((t & 63) == 0) red[t >> 6] = s;
+  __syncthreads();
+  if (t == 0) {
+    const double d = red[0] + red[1] + red[2] + red[3];
+    mom[(long)c * n_mom + m] = mode ? 2.0 * d - mom[(long)c * n_mom + ref] : d;
+  }
+}
+
+// Chebyshev step of chain c (blockIdx.y): t = (L v - b v) / a with (a, b) =
+// rescale[c]; out = t (first step) or 2 t - vm (out may alias vm).
+__global__ void kpm_combine(const double2* __restrict__ Lv, const double2* __restrict__ v,
+                            const double2* vm, const double* __restrict__ rescale, int ld,
+                            int first, double2* out) {
+  const int c = blockIdx.y, e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ld) return;
+  const long i = (long)c * ld + e;
+  const double a = rescale[2 * c], b = rescale[2 * c + 1];
+  const double2 lv = Lv[i], vv = v[i];
+  double2 r = make_double2((lv.x - b * vv.x) / a, (lv.y - b * vv.y) / a);
+  if (!first) {
+    const double2 p = vm[i];
+    r = make_double2(2.0 * r.x - p.x, 2.0 * r.y - p.y);
+  }
+  out[i] = r;
+}
+
 }  // namespace tdscha
 
 using namespace tdscha;
@@ -1530,6 +1573,176 @@ int tdscha_qspace_lanczos_run(tdscha_qctx* ctx, int n_chains, const int* iq_pert
 
 // Time one L application of the batch for every (weights tile, d2v chunk)
 // candidate and every reorthogonalisation tile; keep the fastest.
+int tdscha_qspace_kpm_run(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                          const int* q_pair_map, const double* rescale, int n_moments,
+                          int n_have, double* moments, double* vm, double* v,
+                          int ignore_harmonic, int ignore_v3, int ignore_v4, int use_graph,
+                          double* elapsed_ms) {
+  TDS_REQUIRE(ctx && iq_pert && q_pair_map && rescale && moments && vm && v, 2,
+              "tdscha_qspace_kpm_run: NULL argument");
+  TDS_REQUIRE(n_chains >= 1 && n_chains <= kQMaxChains, 2, "n_chains must be in [1, 256]");
+  TDS_REQUIRE(n_moments >= 1 && n_moments <= 200000, 2, "n_moments must be in [1, 200000]");
+  TDS_REQUIRE(n_have >= 0 && n_have < n_moments, 2, "n_have must be in [0, n_moments)");
+  for (int c = 0; c < n_chains; ++c)
+    TDS_REQUIRE(rescale[2 * c] > 0, 2, "tdscha_qspace_kpm_run: rescale a must be positive");
+  QCtx* q = reinterpret_cast<QCtx*>(ctx);
+  QBatch b;
+  int rc = build_batch(q, n_chains, iq_pert, q_pair_map, b);
+  if (rc) return rc;
+  const int C = n_chains, ld = b.ld, N = n_moments;
+  const size_t nv = (size_t)C * ld;
+  double2 *B0 = nullptr, *B1 = nullptr, *B2 = nullptr, *LV = nullptr;
+  double *dmask = nullptr, *dmom = nullptr, *dres = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto release = [&]() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    void* p[] = {B0, B1, B2, LV, dmask, dmom, dres};
+    for (void* x : p)
+      if (x) (void)hipFree(x);
+    free_batch(b);
+  };
+  auto fail = [&](const char* what) {
+    release();
+    set_error(std::string("tdscha_qspace_kpm_run: ") + what);
+    return -1;
+  };
+  bool ok = hipMalloc((void**)&B0, sizeof(double2) * nv) == hipSuccess &&
+            hipMalloc((void**)&B1, sizeof(double2) * nv) == hipSuccess &&
+            hipMalloc((void**)&B2, sizeof(double2) * nv) == hipSuccess &&
+            hipMalloc((void**)&LV, sizeof(double2) * nv) == hipSuccess &&
+            hipMalloc((void**)&dmask, sizeof(double) * nv) == hipSuccess &&
+            hipMalloc((void**)&dmom, sizeof(double) * C * N) == hipSuccess &&
+            hipMalloc((void**)&dres, sizeof(double) * 2 * C) == hipSuccess;
+  if (!ok) return fail("device allocation failed");
+  // vectors in the padded [C][ld] layout; masks; moments so far
+  std::vector<double2> hvm(nv, make_double2(0.0, 0.0)), hv(nv, make_double2(0.0, 0.0));
+  std::vector<double> hmask(nv, 0.0), hmom((size_t)C * N, 0.0);
+  long pos = 0;
+  for (int c = 0; c < C; ++c) {
+    for (int e = 0; e < b.h_len[c]; ++e) {
+      hvm[(size_t)c * ld + e] = make_double2(vm[2 * (pos + e)], vm[2 * (pos + e) + 1]);
+      hv[(size_t)c * ld + e] = make_double2(v[2 * (pos + e)], v[2 * (pos + e) + 1]);
+    }
+    pos += b.h_len[c];
+    host_mask(b, q->nq, q->nb, q_pair_map, c, hmask);
+    for (int m = 0; m < n_have; ++m) hmom[(size_t)c * N + m] = moments[(size_t)c * N + m];
+    hmom[(size_t)c * N] = n_have ? hmom[(size_t)c * N] : 1.0;
+  }
+  ok = hipMemcpy(B0, hvm.data(), sizeof(double2) * nv, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(B1, hv.data(), sizeof(double2) * nv, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(dmask, hmask.data(), sizeof(double) * nv, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(dmom, hmom.data(), sizeof(double) * C * N, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemcpy(dres, rescale, sizeof(double) * 2 * C, hipMemcpyHostToDevice) == hipSuccess &&
+       hipMemset(B2, 0, sizeof(double2) * nv) == hipSuccess &&
+       hipMemset(LV, 0, sizeof(double2) * nv) == hipSuccess;
+  if (!ok) return fail("upload failed");
+  hipStream_t s = q->stream;
+  const dim3 gv((ld + 255) / 256, C);
+  // The recurrence of QSK:301-440: vm, v hold T_{n-1}, T_n; a step computes
+  // mu_{2n} = 2 <v, v> - mu_0, then vp = 2 L~ v - vm (into the third
+  // buffer) and mu_{2n+1} = 2 <vp, v> - mu_1.  All launches are enqueued
+  // here; with use_graph they are captured first and replayed as one graph.
+  double2* pv[3] = {B0, B1, B2};   // pv[0] = vm, pv[1] = v, pv[2] = spare
+  int n = 0;
+  auto L = [&](const double2* in) {
+    launch_L(q, b, in, LV, ignore_harmonic, ignore_v3, ignore_v4, s);
+  };
+  auto rotate = [&]() {             // vm, v = v, vp
+    double2* old_vm = pv[0];
+    pv[0] = pv[1];
+    pv[1] = pv[2];
+    pv[2] = old_vm;
+  };
+  auto enqueue = [&]() {
+    if (n_have == 0) {
+      // fresh start: vm = v = v0 (given in both); T_1 = L~ v0, mu_1 = <v0, T_1>
+      if (N > 1) {
+        L(pv[1]);
+        hipLaunchKernelGGL(kpm_combine, gv, dim3(256), 0, s, LV, pv[1], pv[0], dres, ld, 1, pv[2]);
+        hipLaunchKernelGGL(kpm_moment, dim3(C), dim3(256), 0, s, pv[1], pv[2], dmask, ld, dmom,
+                           N, 1, 0, 0);
+        rotate();
+        n = 1;
+      }
+    } else if (n_have == 1) {
+      L(pv[1]);
+      hipLaunchKernelGGL(kpm_combine, gv, dim3(256), 0, s, LV, pv[1], pv[0], dres, ld, 1, pv[2]);
+      hipLaunchKernelGGL(kpm_moment, dim3(C), dim3(256), 0, s, pv[0], pv[2], dmask, ld, dmom, N,
+                         1, 0, 0);
+      rotate();
+      n = 1;
+    } else if (n_have % 2 == 0) {
+      n = n_have / 2;
+    } else {
+      const int h = n_have / 2;      // the interrupted odd moment
+      L(pv[1]);
+      hipLaunchKernelGGL(kpm_combine, gv, dim3(256), 0, s, LV, pv[1], pv[0], dres, ld, 0, pv[2]);
+      hipLaunchKernelGGL(kpm_moment, dim3(C), dim3(256), 0, s, pv[2], pv[1], dmask, ld, dmom, N,
+                         2 * h + 1, 1, 1);
+      rotate();
+      n = h + 1;
+    }
+    if (N > 1 || n_have > 0) {
+      while (2 * n < N) {
+        hipLaunchKernelGGL(kpm_moment, dim3(C), dim3(256), 0, s, pv[1], pv[1], dmask, ld, dmom,
+                           N, 2 * n, 1, 0);
+        if (2 * n + 1 >= N) break;
+        L(pv[1]);
+        hipLaunchKernelGGL(kpm_combine, gv, dim3(256), 0, s, LV, pv[1], pv[0], dres, ld, 0, pv[2]);
+        hipLaunchKernelGGL(kpm_moment, dim3(C), dim3(256), 0, s, pv[2], pv[1], dmask, ld, dmom,
+                           N, 2 * n + 1, 1, 1);
+        rotate();
+        ++n;
+      }
+    }
+  };
+  if (hipStreamSynchronize(s) != hipSuccess) return fail("upload synchronisation failed");
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail("event creation failed");
+  if (use_graph) {
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return fail("stream capture failed to start");
+    enqueue();
+    if (hipStreamEndCapture(s, &graph) != hipSuccess) return fail("stream capture failed");
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+      return fail("graph instantiation failed");
+    (void)hipEventRecord(e0, s);
+    if (hipGraphLaunch(exec, s) != hipSuccess) return fail("graph launch failed");
+  } else {
+    (void)hipEventRecord(e0, s);
+    enqueue();
+  }
+  (void)hipEventRecord(e1, s);
+  if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+    return fail("step execution failed");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  if (elapsed_ms) *elapsed_ms = ms;
+  // download the moments and the final recurrence vectors (vm, v)
+  ok = hipMemcpy(hmom.data(), dmom, sizeof(double) * C * N, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(hvm.data(), pv[0], sizeof(double2) * nv, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(hv.data(), pv[1], sizeof(double2) * nv, hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) return fail("download failed");
+  std::copy(hmom.begin(), hmom.end(), moments);
+  pos = 0;
+  for (int c = 0; c < C; ++c) {
+    for (int e = 0; e < b.h_len[c]; ++e) {
+      vm[2 * (pos + e)] = hvm[(size_t)c * ld + e].x;
+      vm[2 * (pos + e) + 1] = hvm[(size_t)c * ld + e].y;
+      v[2 * (pos + e)] = hv[(size_t)c * ld + e].x;
+      v[2 * (pos + e) + 1] = hv[(size_t)c * ld + e].y;
+    }
+    pos += b.h_len[c];
+  }
+  release();
+  return 0;
+}
+
 int tdscha_qspace_autotune(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
                            const int* q_pair_map, int reps, int* tiles_out, double* best_ms) {
   TDS_REQUIRE(ctx && iq_pert && q_pair_map, 2, "tdscha_qspace_autotune: NULL argument");

@@ -6,8 +6,10 @@ L~ = (L - b) / a, with v0 the normalised perturbation and M the Wigner
 metric (mask_dot_wigner), two moments per L application through
 mu_{2n} = 2 <t_n, t_n>_M - mu_0 and mu_{2n+1} = 2 <t_{n+1}, t_n>_M - mu_1
 (QSK:260-454), and the spectral function from the damped Chebyshev series
-(QSK:536-559).  Every L application runs in the gfx950 library (the device
-q-space L of QSpaceLanczos).
+(QSK:536-559).  The recurrence runs in the gfx950 library: every step (L
+application, Chebyshev combination, the two metric products) on the device,
+one graph launch per run, for one perturbation or a batch of them
+(run_KPM_batched).
 """
 from __future__ import annotations
 
@@ -131,14 +133,14 @@ class QSpaceKPM(QSpaceLanczos):
         return (self.apply_full_L(vec) - self.kpm_rescale_b * vec) / self.kpm_rescale_a
 
     def run_KPM(self, n_moments, lambda_min=None, lambda_max=None, bound_factor=1.2,
-                edge_buffer=1e-8, verbose=True):
-        """QSK:260-454, including continuation from saved recurrence vectors."""
+                edge_buffer=1e-8, verbose=True, use_graph=True):
+        """QSK:260-454, including continuation from saved recurrence vectors
+        (odd, even or single-moment restarts).  The recurrence runs on the
+        device (tdscha_qspace_kpm_run), in one graph launch."""
         if n_moments < 1:
             raise ValueError("n_moments must be positive")
         if self.psi is None:
             raise ValueError("Prepare a perturbation before running KPM")
-        mask = self.mask_dot_wigner(False)
-        n_L = 0
         existing = self.kpm_n_moments
         can_continue = (existing > 0 and self.kpm_moments is not None
                         and self.kpm_chebyshev_v is not None)
@@ -147,30 +149,14 @@ class QSpaceKPM(QSpaceLanczos):
                 print("KPM: already have {} moments, {} requested. Nothing to do.".format(
                     existing, n_moments))
             return
-        psi0 = self.psi.copy()
         if can_continue:
-            moments = np.zeros(n_moments, dtype=np.float64)
-            moments[:existing] = self.kpm_moments[:existing]
             vm = np.array(self.kpm_chebyshev_vm, np.complex128)
             v = np.array(self.kpm_chebyshev_v, np.complex128)
-            if existing == 1:
-                v1 = self._apply_rescaled_L(v)
-                n_L += 1
-                moments[1] = self._metric_dot(vm, v1, mask)
-                vm, v = v, v1
-                n = 1
-            elif existing % 2 == 0:
-                n = existing // 2
-            else:
-                h = existing // 2
-                vp = 2 * self._apply_rescaled_L(v) - vm
-                n_L += 1
-                moments[2 * h + 1] = 2 * self._metric_dot(vp, v, mask) - moments[1]
-                vm, v = v, vp
-                n = h + 1
-            self.kpm_n_moments = int(n_moments)
+            have = existing
+            prev = np.asarray(self.kpm_moments)[None, :existing]
         else:
-            norm_sq = self._metric_dot(psi0, psi0, mask)
+            mask = self.mask_dot_wigner(False)
+            norm_sq = self._metric_dot(self.psi, self.psi, mask)
             if norm_sq <= EPSILON or np.isnan(norm_sq):
                 raise ValueError("Prepare a non-zero perturbation before running KPM")
             self.kpm_lambda_min, self.kpm_lambda_max = self._get_kpm_bounds(
@@ -178,35 +164,49 @@ class QSpaceKPM(QSpaceLanczos):
             self.kpm_rescale_a = 0.5 * (self.kpm_lambda_max - self.kpm_lambda_min)
             self.kpm_rescale_b = 0.5 * (self.kpm_lambda_max + self.kpm_lambda_min)
             self.kpm_vector_norm = np.sqrt(norm_sq)
-            self.kpm_n_moments = int(n_moments)
             self.kpm_bound_factor = bound_factor
-            v0 = psi0 / self.kpm_vector_norm
-            moments = np.zeros(self.kpm_n_moments, dtype=np.float64)
-            moments[0] = 1.0
-            vm, v = v0, v0
-            n = None
-            if self.kpm_n_moments > 1:
-                v1 = self._apply_rescaled_L(v0)
-                n_L += 1
-                moments[1] = self._metric_dot(v0, v1, mask)
-                vm, v = v0, v1
-                n = 1
-        if n is not None:
-            while 2 * n < self.kpm_n_moments:
-                moments[2 * n] = 2 * self._metric_dot(v, v, mask) - moments[0]
-                if 2 * n + 1 >= self.kpm_n_moments:
-                    break
-                vp = 2 * self._apply_rescaled_L(v) - vm
-                n_L += 1
-                moments[2 * n + 1] = 2 * self._metric_dot(vp, v, mask) - moments[1]
-                vm, v = v, vp
-                n += 1
-        self.psi = psi0
-        self.kpm_chebyshev_vm = vm.copy()
-        self.kpm_chebyshev_v = v.copy()
-        self.kpm_moments = moments
+            vm = v = np.asarray(self.psi, np.complex128) / self.kpm_vector_norm
+            have, prev = 0, None
+        mom, vms, vs, ms = self.engine.kpm(
+            [self.iq_pert], [self.q_pair_map], [vm], [v],
+            [self.kpm_rescale_a, self.kpm_rescale_b], n_moments, have, prev,
+            self.ignore_harmonic, self.ignore_v3, self.ignore_v4, use_graph)
+        self.kpm_n_moments = int(n_moments)
+        self.kpm_moments = mom[0]
+        self.kpm_chebyshev_vm, self.kpm_chebyshev_v = vms[0], vs[0]
         if verbose:
-            print("KPM completed: {} moments, {} L applications".format(self.kpm_n_moments, n_L))
+            print("KPM completed: {} moments in {:.2f} ms on the device".format(n_moments, ms))
+
+    def run_KPM_batched(self, iq_perts, psis, n_moments, lambda_min=None, lambda_max=None,
+                        bound_factor=1.2, edge_buffer=1e-8, use_graph=True):
+        """Moments of many perturbations (each at its own q-point) in one
+        device run.  Returns (moments [C, n_moments], norms, (rescale_a,
+        rescale_b), device ms); the spectra follow from chebyshev_spectrum."""
+        from .qspace import pair_map
+        lmin, lmax = self._get_kpm_bounds(lambda_min, lambda_max, bound_factor, edge_buffer)
+        a, b = 0.5 * (lmax - lmin), 0.5 * (lmax + lmin)
+        pms, v0s, norms = [], [], []
+        for iq, psi in zip(iq_perts, psis):
+            pm, pairs = pair_map(self.L, int(iq))
+            nb = self.n_bands
+            mask = np.concatenate([np.ones(nb)] + 2 * [
+                np.full(nb * nb, 2.0) if p < q else
+                np.where(np.triu_indices(nb)[0] == np.triu_indices(nb)[1], 1.0, 2.0)
+                for p, q in pairs])
+            psi = np.asarray(psi, np.complex128)
+            if len(psi) != len(mask):
+                raise ValueError("psi of q-point %d has %d entries, its layout %d"
+                                 % (iq, len(psi), len(mask)))
+            nrm = np.sqrt(float(np.real(np.conj(psi).dot(psi * mask))))
+            if nrm <= np.sqrt(EPSILON):
+                raise ValueError("Prepare a non-zero perturbation before running KPM")
+            pms.append(pm)
+            v0s.append(psi / nrm)
+            norms.append(nrm)
+        mom, _, _, ms = self.engine.kpm(list(iq_perts), pms, v0s, v0s, [(a, b)] * len(pms),
+                                        n_moments, 0, None, self.ignore_harmonic,
+                                        self.ignore_v3, self.ignore_v4, use_graph)
+        return mom, np.array(norms), (a, b), ms
 
     def save_kpm(self, file):
         """QSK:456-471: moments with the parameters in a one-line header."""

@@ -111,6 +111,30 @@ class QSpaceEngine:
             chains.append((a[i, :k].copy(), b[i, :nb].copy(), c[i, :nb].copy()))
         return chains, ms.value
 
+    def kpm(self, iq_pert, pair_maps, vms, vs, rescale, n_moments, n_have=0, moments=None,
+            ignore_harmonic=False, ignore_v3=False, ignore_v4=False, use_graph=True):
+        """Device Chebyshev recurrence of many chains (tdscha_qspace_kpm_run):
+        returns (moments [C, n_moments], vms, vs, device ms)."""
+        iqs = np.ascontiguousarray(iq_pert, np.int32)
+        pms = np.ascontiguousarray(pair_maps, np.int32)
+        nc = len(iqs)
+        sizes = [len(x) for x in vs]
+        fvm = np.ascontiguousarray(np.concatenate(vms), np.complex128)
+        fv = np.ascontiguousarray(np.concatenate(vs), np.complex128)
+        res = np.ascontiguousarray(rescale, np.float64).reshape(nc, 2)
+        mom = np.zeros((nc, n_moments))
+        if n_have:
+            mom[:, :n_have] = np.asarray(moments)[:, :n_have]
+        ms = C.c_double(0.0)
+        _lib.check(_lib.lib().tdscha_qspace_kpm_run(
+            self.ctx, nc, _p(iqs), _p(pms), _p(res), int(n_moments), int(n_have), _p(mom),
+            _p(fvm.view(np.float64)), _p(fv.view(np.float64)), int(ignore_harmonic),
+            int(ignore_v3), int(ignore_v4), int(bool(use_graph)), C.byref(ms)),
+            "tdscha_qspace_kpm_run")
+        cut = np.cumsum([0] + sizes)
+        return (mom, [fvm[cut[i]:cut[i + 1]].copy() for i in range(nc)],
+                [fv[cut[i]:cut[i + 1]].copy() for i in range(nc)], ms.value)
+
     def set_tiles(self, weights_tile, d2v_chunk, reorth_jtile):
         _lib.check(_lib.lib().tdscha_qspace_set_tiles(self.ctx, int(weights_tile),
                                                       int(d2v_chunk), int(reorth_jtile)),
