@@ -167,6 +167,25 @@ def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True, tiles
            "autotune_best_L_ms": tune_ms,
            "setup_s": t_setup, "chains_stopped_on_tol": int(sum(len(b) < len(a)
                                                                for a, b, _ in chains))}
+    # kernel polynomial method on the same batch: 256 Chebyshev moments per
+    # chain (128 L applications), one graph launch
+    from tdscha_amd.qkpm import kpm_bounds
+    from tdscha_amd.qlanczos import QSpaceLanczos
+    lmin, lmax = kpm_bounds(modes.w_q, modes.valid_modes_q, True, bound_factor=1.2)
+    ql = QSpaceLanczos()
+    ql.n_bands, ql.L = nb, modes.L
+    v0s = []
+    for iq, psi in zip(iqs, psis):
+        ql.build_q_pair_map(int(iq))
+        v0s.append(psi / np.sqrt(np.real(np.conj(psi) @ (psi * ql.mask_dot_wigner()))))
+    n_mom = 256
+    t = time.perf_counter()
+    _, _, _, kpm_ms = eng.kpm(iqs, pms, v0s, v0s, [(0.5 * (lmax - lmin), 0.5 * (lmax + lmin))] * nq,
+                              n_mom)
+    kpm_wall = time.perf_counter() - t
+    out.update({"kpm_chains": nq, "kpm_moments": n_mom,
+                "kpm_L_applications_per_s": nq * (n_mom // 2) / kpm_wall,
+                "kpm_device_ms": kpm_ms, "kpm_wall_s": kpm_wall})
     if per_chain:
         tiles1, _ = eng.autotune(iqs[:1], pms[:1], reps=2)     # tiles for one chain at a time
         t = time.perf_counter()
