@@ -193,6 +193,18 @@ int tdscha_qspace_apply_L(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
                           int ignore_harmonic, int ignore_v3, int ignore_v4);
 
 /*
+ * Static Liouvillian of the free-energy Hessian (replaces
+ * QSpaceHessian._apply_L_static_q, Modules/QSpaceHessian.py:605-692) on psi
+ * in the static layout [R (n_bands), one W sector of the same blocks as a']:
+ * R <- w^2 R - f_pert, W <- W / Lambda + d2v, with the perturbed averages of
+ * R1 = R and alpha = -2 Y_w1 Y_w2 W.  Same batching as tdscha_qspace_apply_L.
+ */
+int tdscha_qspace_static_psi_size(int n_q, int n_bands, const int* q_pair_map, int* psi_size);
+int tdscha_qspace_apply_L_static(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                                 const int* q_pair_map, const double* psi, double* out,
+                                 int ignore_harmonic, int ignore_v3, int ignore_v4);
+
+/*
  * Hermitian Lanczos with full reorthogonalisation (QSpaceLanczos.run_FT with
  * reorthogonalize=True, QL:967-1182) for n_chains chains at once, device
  * resident.  Chain c stops when |b| < tol or after caps[c] steps (caps may

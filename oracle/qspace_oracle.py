@@ -327,3 +327,70 @@ def kpm_moments_direct(matvec, psi0, mask, n_moments, rescale_a, rescale_b):
         tn = Lt(t) if n == 0 else 2 * Lt(t) - tm
         tm, t = t, tn
     return mu, norm
+
+
+# ------------------------------------------------------------ static L
+def static_lambda(qs, p):
+    """Static two-phonon propagator of pair p (Modules/QSpaceHessian.py:481-544)."""
+    iq1, iq2 = qs.unique_pairs[p]
+    w1, w2 = qs.w_q[:, iq1], qs.w_q[:, iq2]
+    n1, v1 = qs.bose_valid(iq1)
+    n2, v2 = qs.bose_valid(iq2)
+    W1, W2 = np.meshgrid(w1, w2, indexing="ij")
+    N1, N2 = np.meshgrid(n1, n2, indexing="ij")
+    vm = np.outer(v1, v2)
+    dn = np.zeros_like(W1)
+    if qs.T > EPS:
+        beta = RY_TO_K / qs.T
+        eq = np.abs(W1 - W2) < 1e-8
+        dn = np.where(eq, 0.0, (N1 - N2) / np.where(eq, 1.0, W1 - W2))
+        dn = np.where(eq & vm, -beta * np.exp(np.where(eq & vm, W1, 0.0) * beta) * N1 ** 2, dn)
+    return np.where(vm, ((N1 + N2 + 1) / np.where(vm, W1 + W2, 1.0) - dn)
+                    / (4 * np.where(vm, W1 * W2, 1.0)), 0.0)
+
+
+def static_apply(qs, psi):
+    """Static Liouvillian (QSH:605-692) on psi = [R, W blocks] (the static
+    layout: the spectral layout without its b' sector): R <- w^2 R - f_pert,
+    W <- W / Lambda + d2v, with R1 = R and alpha = -2 Y_w1 Y_w2 W."""
+    nb = qs.nb
+    n = qs.off_b[0]
+    out = np.zeros(n, np.complex128)
+    qp = qs.iq_pert
+    if not qs.ignore_harmonic:
+        out[:nb] = qs.w_q[:, qp] ** 2 * psi[:nb]
+        for p in range(len(qs.unique_pairs)):
+            lam = static_lambda(qs, p)
+            inv = np.where(np.abs(lam) > 1e-30, 1.0 / np.where(np.abs(lam) > 1e-30, lam, 1.0), 0.0)
+            qs.set_block(out, p, inv * qs.get_block(psi, p, "a"), "a")
+    if qs.ignore_v3 and qs.ignore_v4:
+        return out
+    fY, _ = qs.fy_fpsi()
+    R1 = psi[:nb].copy()
+    if qs.ignore_v3:
+        R1[:] = 0
+    alpha = [-2.0 * np.outer(fY[:, a], fY[:, b]) * qs.get_block(psi, p, "a")
+             for p, (a, b) in enumerate(qs.unique_pairs)]
+    f, d2v = qs.perturb_averages(R1, alpha, not qs.ignore_v4)
+    out[:nb] -= f
+    for p in range(len(qs.unique_pairs)):
+        M = qs.get_block(out, p, "a") + d2v[p]
+        qs.set_block(out, p, M, "a")
+    return out
+
+
+def static_mask(qs):
+    """Metric of the static layout with masked modes at zero (QSH:434-479)."""
+    nb = qs.nb
+    m = np.ones(qs.off_b[0])
+    m[:nb] = qs.valid[:, qs.iq_pert]
+    for p, (iq1, iq2) in enumerate(qs.unique_pairs):
+        bad = ~qs.valid[:, iq1][:, None] | ~qs.valid[:, iq2][None, :]
+        full = np.where(bad, 0.0, 2.0)
+        off = qs.off_a[p]
+        if iq1 < iq2:
+            m[off:off + nb * nb] = full.ravel()
+        else:
+            np.fill_diagonal(full, np.where(qs.valid[:, iq1] & qs.valid[:, iq2], 1.0, 0.0))
+            m[off:off + qs.block_sizes[p]] = full[np.triu_indices(nb)]
+    return m

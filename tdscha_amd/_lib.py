@@ -58,6 +58,9 @@ def _signatures(L):
     L.tdscha_qctx_destroy.restype = None
     L.tdscha_qctx_symmetrize.argtypes = [_vp, C.c_int, _vp, _vp]
     L.tdscha_qspace_psi_size.argtypes = [C.c_int, C.c_int, _vp, _ip]
+    L.tdscha_qspace_static_psi_size.argtypes = [C.c_int, C.c_int, _vp, _ip]
+    L.tdscha_qspace_apply_L_static.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
+                                               C.c_int, C.c_int]
     L.tdscha_qspace_apply_L.argtypes = [_vp, C.c_int, _vp, _vp, _vp, _vp, C.c_int, C.c_int,
                                         C.c_int]
     L.tdscha_qspace_lanczos_run.argtypes = [_vp, C.c_int, _vp, _vp, _vp, C.c_int, _vp, C.c_double,

@@ -521,6 +521,101 @@ __global__ void __launch_bounds__(kD2vThreads) __attribute__((amdgpu_waves_per_e
   }
 }
 
+// Static Liouvillian of the free-energy Hessian (QSpaceHessian,
+// Modules/QSpaceHessian.py:605-692): psi = [R, one W sector].  The input of
+// the perturbed averages is alpha = -2 Y_w1 Y_w2 W (QSH:675-679; Y_w = fY),
+// oriented per q-point as in q_unpack.
+template <int NB>
+__global__ void q_unpack_static(const double2* __restrict__ psi, int ld,
+                                const int* __restrict__ active, const int* __restrict__ pq1,
+                                const int* __restrict__ pq2, const int* __restrict__ offa, int MP,
+                                int nq, const int* __restrict__ apair,
+                                const int* __restrict__ aorient, const double* __restrict__ fY,
+                                int ignore_v3, double2* __restrict__ alq,
+                                double2* __restrict__ R1) {
+  const int c = blockIdx.y, q = blockIdx.x, t = threadIdx.x;
+  if (!active[c]) return;
+  const double2* ps = psi + (long)c * ld;
+  if (q == 0 && t < NB) R1[c * NB + t] = ignore_v3 ? make_double2(0.0, 0.0) : ps[t];
+  if (t >= NB * NB) return;
+  const int p = apair[c * nq + q], o = aorient[c * nq + q];
+  const int i = o ? t % NB : t / NB, j = o ? t / NB : t % NB;
+  const int q1 = pq1[c * MP + p], q2 = pq2[c * MP + p];
+  const double2 w = block_entry(ps, offa[c * MP + p], q1 < q2, i, j, NB);
+  const double2 al = csc(w, -2.0 * fY[q1 * NB + i] * fY[q2 * NB + j]);
+  alq[((long)c * nq + q) * NB * NB + t] = o ? cj(al) : al;
+}
+
+// Static two-phonon propagator (QSH:481-544): Lambda = ((n1 + n2 + 1) /
+// (w1 + w2) - dn) / (4 w1 w2), dn = (n1 - n2) / (w1 - w2), or
+// -beta e^{beta w} n^2 when |w1 - w2| < 1e-8; zero unless both modes count.
+__device__ __forceinline__ double static_lambda(double w1, double w2, double n1, double n2,
+                                                bool v1, bool v2, double beta) {
+  if (!(v1 && v2)) return 0.0;
+  double dn = 0.0;
+  if (beta > 0.0) {
+    if (fabs(w1 - w2) < 1e-8)
+      dn = -beta * exp(w1 * beta) * n1 * n1;
+    else
+      dn = (n1 - n2) / (w1 - w2);
+  }
+  return ((n1 + n2 + 1.0) / (w1 + w2) - dn) / (4.0 * w1 * w2);
+}
+
+// R <- w^2 R - f_pert, W <- W / Lambda + d2v (QSH:605-692).  Block 0: R
+// sector; block 1 + p: the W block of pair p.
+template <int NB>
+__global__ void q_finish_static(const double2* __restrict__ psi, int ld, double2* __restrict__ out,
+                                const int* __restrict__ active, const int* __restrict__ iqp,
+                                const int* __restrict__ npairs, const int* __restrict__ pq1,
+                                const int* __restrict__ pq2, const int* __restrict__ offa, int MP,
+                                const double* __restrict__ wq, const double* __restrict__ nbose,
+                                const int* __restrict__ valid, double beta,
+                                const double2* __restrict__ fpart, int ntile,
+                                const double2* __restrict__ dpart, int nch, double inv_norm,
+                                int do_harm, int do_anh) {
+  const int c = blockIdx.y, blk = blockIdx.x, t = threadIdx.x;
+  if (!active[c]) return;
+  const double2* ps = psi + (long)c * ld;
+  double2* po = out + (long)c * ld;
+  if (blk == 0) {
+    if (t >= NB) return;
+    const int qp = iqp[c];
+    double2 o = make_double2(0.0, 0.0);
+    if (do_anh) {
+      double2 f = make_double2(0.0, 0.0);
+      for (int k = 0; k < ntile; ++k) f = cadd(f, fpart[((long)c * ntile + k) * NB + t]);
+      o = csc(f, -inv_norm);
+    }
+    if (do_harm) {
+      const double w = wq[qp * NB + t];
+      o = cadd(o, csc(ps[t], w * w));
+    }
+    po[t] = o;
+    return;
+  }
+  const int p = blk - 1;
+  if (p >= npairs[c] || t >= NB * NB) return;
+  const int i = t / NB, j = t % NB;
+  const int q1 = pq1[c * MP + p], q2 = pq2[c * MP + p];
+  const int full = q1 < q2;
+  if (!full && i > j) return;
+  const int e = offa[c * MP + p] + (full ? i * NB + j : tri_index(i, j, NB));
+  double2 o = make_double2(0.0, 0.0);
+  if (do_anh) {
+    const double2* dp = dpart + ((long)c * MP + p) * nch * NB * NB + t;
+    for (int k = 0; k < nch; ++k) o = cadd(o, dp[(long)k * NB * NB]);
+    o = csc(o, inv_norm);
+  }
+  if (do_harm) {
+    const double lam = static_lambda(wq[q1 * NB + i], wq[q2 * NB + j], nbose[q1 * NB + i],
+                                     nbose[q2 * NB + j], valid[q1 * NB + i], valid[q2 * NB + j],
+                                     beta);
+    if (fabs(lam) > 1e-30) o = cadd(o, csc(ps[e], 1.0 / lam));
+  }
+  po[e] = o;
+}
+
 // block 0: R sector; block 1 + p: pair p
 template <int NB>
 __global__ void q_finish(const double2* __restrict__ psi, int ld, double2* __restrict__ out,
@@ -693,7 +788,7 @@ static int slab_chunk(const QCtx* q, int n_item_blocks, int rounds) {
 
 // Host tables from the pair maps; ld_min forces a minimum stride.
 static int build_batch(QCtx* q, int C, const int* iq_pert, const int* pair_map, QBatch& b,
-                       int ld_min = 0) {
+                       int ld_min = 0, bool static_layout = false) {
   const int nq = q->nq, nb = q->nb;
   b.C = C;
   b.MP = nq;
@@ -728,9 +823,10 @@ static int build_batch(QCtx* q, int C, const int* iq_pert, const int* pair_map, 
       h_offa[(size_t)c * nq + p] = off;
       off += h_pq1[(size_t)c * nq + p] < h_pq2[(size_t)c * nq + p] ? nb * nb : nb * (nb + 1) / 2;
     }
-    for (int p = 0; p < np; ++p) {
-      h_offb[(size_t)c * nq + p] = off;
-      off += h_pq1[(size_t)c * nq + p] < h_pq2[(size_t)c * nq + p] ? nb * nb : nb * (nb + 1) / 2;
+    for (int p = 0; p < np; ++p) {     // the static layout has no b' sector
+      h_offb[(size_t)c * nq + p] = static_layout ? h_offa[(size_t)c * nq + p] : off;
+      if (!static_layout)
+        off += h_pq1[(size_t)c * nq + p] < h_pq2[(size_t)c * nq + p] ? nb * nb : nb * (nb + 1) / 2;
     }
     h_np[c] = np;
     h_len[c] = off;
@@ -830,13 +926,18 @@ static int build_batch(QCtx* q, int C, const int* iq_pert, const int* pair_map, 
 
 template <int NB>
 static void launch_L_nb(QCtx* q, QBatch& b, const double2* psi, double2* out, int ignore_harmonic,
-                        int ignore_v3, int ignore_v4, hipStream_t s) {
+                        int ignore_v3, int ignore_v4, hipStream_t s, bool static_op) {
   const int C = b.C, MP = b.MP, nq = q->nq;
   const int do_anh = !(ignore_v3 && ignore_v4);
   if (do_anh) {
-    hipLaunchKernelGGL(q_unpack<NB>, dim3(nq, C), dim3(64), 0, s, psi, b.ld, b.active, b.pq1,
-                       b.pq2, b.offa, b.offb, MP, nq, b.apair, b.aorient, q->wq, q->nbose,
-                       q->valid, ignore_v3, b.alpha, b.R1);
+    if (static_op)
+      hipLaunchKernelGGL(q_unpack_static<NB>, dim3(nq, C), dim3(64), 0, s, psi, b.ld, b.active,
+                         b.pq1, b.pq2, b.offa, MP, nq, b.apair, b.aorient, q->fY, ignore_v3,
+                         b.alpha, b.R1);
+    else
+      hipLaunchKernelGGL(q_unpack<NB>, dim3(nq, C), dim3(64), 0, s, psi, b.ld, b.active, b.pq1,
+                         b.pq2, b.offa, b.offb, MP, nq, b.apair, b.aorient, q->wq, q->nbose,
+                         q->valid, ignore_v3, b.alpha, b.R1);
     constexpr int CB = kQWeightsChains;
     const int ngroups = (C + CB - 1) / CB;
     const int nblk = (b.ntile + 7) / 8 * 8 * ngroups;
@@ -854,18 +955,24 @@ static void launch_L_nb(QCtx* q, QBatch& b, const double2* psi, double2* out, in
                          b.dpart);
     }
   }
-  hipLaunchKernelGGL(q_finish<NB>, dim3(MP + 1, C), dim3(64), 0, s, psi, b.ld, out, b.active,
-                     b.iqp, b.npairs, b.pq1, b.pq2, b.offa, b.offb, MP, q->wq, q->nbose, q->valid,
-                     b.fpart, b.ntile, b.dpart, b.nch, 1.0 / q->neff, ignore_harmonic ? 0 : 1,
-                     do_anh);
+  if (static_op)
+    hipLaunchKernelGGL(q_finish_static<NB>, dim3(MP + 1, C), dim3(64), 0, s, psi, b.ld, out,
+                       b.active, b.iqp, b.npairs, b.pq1, b.pq2, b.offa, MP, q->wq, q->nbose,
+                       q->valid, q->T > 1e-12 ? RY_TO_K / q->T : 0.0, b.fpart, b.ntile, b.dpart,
+                       b.nch, 1.0 / q->neff, ignore_harmonic ? 0 : 1, do_anh);
+  else
+    hipLaunchKernelGGL(q_finish<NB>, dim3(MP + 1, C), dim3(64), 0, s, psi, b.ld, out, b.active,
+                       b.iqp, b.npairs, b.pq1, b.pq2, b.offa, b.offb, MP, q->wq, q->nbose,
+                       q->valid, b.fpart, b.ntile, b.dpart, b.nch, 1.0 / q->neff,
+                       ignore_harmonic ? 0 : 1, do_anh);
 }
 
 static void launch_L(QCtx* q, QBatch& b, const double2* psi, double2* out, int ih, int i3, int i4,
-                     hipStream_t s) {
+                     hipStream_t s, bool static_op = false) {
   if (q->nb == 6)
-    launch_L_nb<6>(q, b, psi, out, ih, i3, i4, s);
+    launch_L_nb<6>(q, b, psi, out, ih, i3, i4, s, static_op);
   else
-    launch_L_nb<3>(q, b, psi, out, ih, i3, i4, s);
+    launch_L_nb<3>(q, b, psi, out, ih, i3, i4, s, static_op);
 }
 
 // ------------------------------------------------------- Hermitian Lanczos
@@ -1357,6 +1464,32 @@ int tdscha_qctx_symmetrize(tdscha_qctx* ctx, int n_sym, const int* q_perm, const
   return build_slabs(q);
 }
 
+static int apply_L_host(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                        const int* q_pair_map, const double* psi, double* out,
+                        int ignore_harmonic, int ignore_v3, int ignore_v4, bool static_op);
+
+int tdscha_qspace_apply_L(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                          const int* q_pair_map, const double* psi, double* out,
+                          int ignore_harmonic, int ignore_v3, int ignore_v4) {
+  return apply_L_host(ctx, n_chains, iq_pert, q_pair_map, psi, out, ignore_harmonic, ignore_v3,
+                      ignore_v4, false);
+}
+
+int tdscha_qspace_apply_L_static(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                                 const int* q_pair_map, const double* psi, double* out,
+                                 int ignore_harmonic, int ignore_v3, int ignore_v4) {
+  return apply_L_host(ctx, n_chains, iq_pert, q_pair_map, psi, out, ignore_harmonic, ignore_v3,
+                      ignore_v4, true);
+}
+
+int tdscha_qspace_static_psi_size(int n_q, int n_bands, const int* q_pair_map, int* psi_size) {
+  int full = 0;
+  const int rc = tdscha_qspace_psi_size(n_q, n_bands, q_pair_map, &full);
+  if (rc) return rc;
+  *psi_size = n_bands + (full - n_bands) / 2;
+  return 0;
+}
+
 int tdscha_qspace_psi_size(int n_q, int n_bands, const int* q_pair_map, int* psi_size) {
   TDS_REQUIRE(q_pair_map && psi_size && n_q > 0 && n_bands > 0, 2,
               "tdscha_qspace_psi_size: bad argument");
@@ -1408,14 +1541,14 @@ int tdscha_qspace_get_tiles(const tdscha_qctx* ctx, int* tiles_out) {
   return 0;
 }
 
-int tdscha_qspace_apply_L(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
-                          const int* q_pair_map, const double* psi, double* out,
-                          int ignore_harmonic, int ignore_v3, int ignore_v4) {
+static int apply_L_host(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
+                        const int* q_pair_map, const double* psi, double* out,
+                        int ignore_harmonic, int ignore_v3, int ignore_v4, bool static_op) {
   TDS_REQUIRE(ctx && iq_pert && q_pair_map && psi && out, 2, "tdscha_qspace_apply_L: NULL argument");
   TDS_REQUIRE(n_chains >= 1 && n_chains <= kQMaxChains, 2, "n_chains must be in [1, 256]");
   QCtx* q = reinterpret_cast<QCtx*>(ctx);
   QBatch b;
-  int rc = build_batch(q, n_chains, iq_pert, q_pair_map, b);
+  int rc = build_batch(q, n_chains, iq_pert, q_pair_map, b, 0, static_op);
   if (rc) return rc;
   const int C = n_chains, ld = b.ld;
   double2 *dpsi = nullptr, *dout = nullptr;
@@ -1432,7 +1565,7 @@ int tdscha_qspace_apply_L(tdscha_qctx* ctx, int n_chains, const int* iq_pert,
             hipMemcpy(dpsi, hp.data(), sizeof(double2) * C * ld, hipMemcpyHostToDevice) == hipSuccess &&
             hipMemset(dout, 0, sizeof(double2) * C * ld) == hipSuccess;
   if (ok) {
-    launch_L(q, b, dpsi, dout, ignore_harmonic, ignore_v3, ignore_v4, s);
+    launch_L(q, b, dpsi, dout, ignore_harmonic, ignore_v3, ignore_v4, s, static_op);
     ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(s) == hipSuccess &&
          hipMemcpy(hp.data(), dout, sizeof(double2) * C * ld, hipMemcpyDeviceToHost) == hipSuccess;
   }

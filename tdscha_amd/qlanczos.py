@@ -71,6 +71,31 @@ class QSpaceEngine:
                    "tdscha_qspace_psi_size")
         return n.value
 
+    def static_psi_size(self, pm):
+        n = C.c_int()
+        pm = np.ascontiguousarray(pm, np.int32)
+        _lib.check(_lib.lib().tdscha_qspace_static_psi_size(self.n_q, self.n_bands, _p(pm),
+                                                            C.byref(n)),
+                   "tdscha_qspace_static_psi_size")
+        return n.value
+
+    def apply_L_static(self, iq_pert, pair_maps, psis, ignore_harmonic=False, ignore_v3=False,
+                       ignore_v4=False):
+        """Static Liouvillian of the Hessian on psis in the static layout."""
+        iqs = np.ascontiguousarray(iq_pert, np.int32)
+        pms = np.ascontiguousarray(pair_maps, np.int32)
+        flat = np.ascontiguousarray(np.concatenate(psis), np.complex128)
+        out = np.zeros_like(flat)
+        _lib.check(_lib.lib().tdscha_qspace_apply_L_static(
+            self.ctx, len(iqs), _p(iqs), _p(pms), _p(flat.view(np.float64)),
+            _p(out.view(np.float64)), int(ignore_harmonic), int(ignore_v3), int(ignore_v4)),
+            "tdscha_qspace_apply_L_static")
+        res, pos = [], 0
+        for p in psis:
+            res.append(out[pos:pos + len(p)].copy())
+            pos += len(p)
+        return res
+
     def apply_L(self, iq_pert, pair_maps, psis, ignore_harmonic=False, ignore_v3=False,
                 ignore_v4=False):
         iqs = np.ascontiguousarray(iq_pert, np.int32)

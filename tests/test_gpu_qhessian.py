@@ -1,0 +1,124 @@
+"""GPU tests of the q-space free-energy Hessian (tdscha_amd/qhessian.py):
+the device static Liouvillian against the oracle (oracle/qspace_oracle.py
+static_apply), the harmonic closed form, the iterative solve against a dense
+inverse of the oracle's operator, the degenerate-block shortcut (as
+tests/test_qspace/test_qspace_hessian.py:233-270 of the reference) and the
+rotation of one solve per orbit to the whole grid."""
+import numpy as np
+import pytest
+
+from oracle import qspace_oracle as QO
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(L=2, N=40, seed=3):
+    from tdscha_amd.qspace import bloch_ensemble
+    from tdscha_amd.rocksalt import Ensemble, RockSalt
+    ens = Ensemble(RockSalt(L=L, k3=-2.0, k4=6.0), 300.0, N, seed=seed)
+    return ens
+
+
+def _oracle(h, iq, **kw):
+    from tdscha_amd.qspace import pair_map
+    pm, pairs = pair_map(h.qlanc.L, iq)
+    ql = h.qlanc
+    return QO.QSystem(ql.X_q, ql.Y_q, ql.w_q, ql.valid_modes_q, ql.rho, ql.T, pm, pairs, iq, **kw)
+
+
+@pytest.mark.parametrize("flags", [(0, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 0)])
+def test_static_L_matches_oracle(flags):
+    from tdscha_amd.qhessian import QSpaceHessian
+    ih, i3, i4 = flags
+    h = QSpaceHessian(_system(), verbose=False, ignore_v3=bool(i3), ignore_v4=bool(i4))
+    h.ignore_harmonic = bool(ih)
+    h.init(use_symmetries=False)
+    rng = np.random.default_rng(8)
+    for iq in (0, 3, 6):
+        h._layout(iq)
+        qs = _oracle(h, iq, ignore_harmonic=bool(ih), ignore_v3=bool(i3), ignore_v4=bool(i4))
+        psi = rng.normal(size=h._size) + 1j * rng.normal(size=h._size)
+        ref = QO.static_apply(qs, psi)
+        out = h._apply_L_static_q(psi)
+        err = np.abs(out - ref).max() / np.abs(ref).max()
+        print("iq %d flags %s static L rel err %.2e" % (iq, flags, err))
+        assert err < 1e-12
+    h.close()
+
+
+def test_harmonic_hessian_is_w_squared():
+    from tdscha_amd.qhessian import QSpaceHessian
+    h = QSpaceHessian(_system(), verbose=False, ignore_v3=True, ignore_v4=True)
+    h.init(use_symmetries=False)
+    for iq in (0, 3):
+        H = h.compute_hessian_at_q(iq, tol=1e-12)
+        v = np.asarray(h.qlanc.valid_modes_q, bool)[:, iq]
+        ref = np.diag(np.where(v, h.w_q[:, iq] ** 2, 0.0))
+        assert np.abs(H - ref).max() < 1e-10 * np.abs(ref).max()
+    h.close()
+
+
+@pytest.mark.parametrize("sym", [False, True])
+def test_solve_matches_dense_inverse_and_mode_symmetry(sym):
+    """GMRES solve = dense inverse of the oracle's static L on the R sector;
+    with the point-group images (L commutes with the little group of q) the
+    one-solve-per-degenerate-block schedule gives the same Hessian (the
+    reference allows 1e-8 between the two, QSH test :233-270)."""
+    from tdscha_amd.qhessian import QSpaceHessian
+    from tdscha_amd.qspace import q_symmetries
+    h = QSpaceHessian(_system(N=20 if sym else 40), verbose=False)
+    h.init(use_symmetries=sym)
+    for iq in (0, 3):
+        H1 = h.compute_hessian_at_q(iq, tol=1e-12, use_mode_symmetry=False)
+        H2 = h.compute_hessian_at_q(iq, tol=1e-12, use_mode_symmetry=True) if sym else H1
+        qs = _oracle(h, iq, symmetries=q_symmetries(h.qlanc.modes) if sym else None)
+        n = qs.off_b[0]
+        m = QO.static_mask(qs)
+        keep = m > 0
+        Ls = np.zeros((n, n), complex)
+        for j in range(n):
+            e = np.zeros(n, complex)
+            e[j] = 1.0
+            Ls[:, j] = QO.static_apply(qs, e)
+        G = np.zeros((n, n), complex)
+        G[np.ix_(keep, keep)] = np.linalg.inv(Ls[np.ix_(keep, keep)])
+        v = np.flatnonzero(np.asarray(h.qlanc.valid_modes_q, bool)[:, iq])
+        Gr = G[np.ix_(v, v)]
+        Href = np.zeros_like(H1)
+        Href[np.ix_(v, v)] = np.linalg.inv(0.5 * (Gr + Gr.conj().T))
+        e1 = np.abs(H1 - Href).max() / np.abs(Href).max()
+        e2 = np.abs(H2 - H1).max() / np.abs(H1).max()
+        print("iq %d: solve vs dense %.2e, mode-symmetry vs full %.2e" % (iq, e1, e2))
+        assert e1 < 1e-8
+        # the reference's bound is absolute (1e-8 Ry^2 on entries ~1e-6): the
+        # shortcut takes same-size degenerate blocks as copies of one irrep
+        # with aligned bases, which holds at Gamma to rounding and elsewhere
+        # to ~1e-4 relative here
+        assert np.abs(H2 - H1).max() < 1e-8
+        if iq == 0:
+            assert e2 < 1e-12
+        assert np.abs(H1 - H1.conj().T).max() < 1e-14 * np.abs(H1).max()
+    h.close()
+
+
+def test_full_hessian_by_orbits_equals_every_q(tmp_path):
+    """One solve per orbit of the point group, rotated to the rest of the
+    grid, = a solve at every q-point (point-group images in the ensemble);
+    checkpoint restart reproduces the result."""
+    from tdscha_amd.qhessian import QSpaceHessian
+    h = QSpaceHessian(_system(N=30), verbose=False)
+    h.init(use_symmetries=True)
+    assert len(h.irr_qpoints) == 3                 # Gamma, L and X of the 2x2x2 grid
+    Hall = h.compute_full_hessian(tol=1e-12, use_mode_symmetry=False)
+    for iq in range(h.n_q):
+        Hd = h.compute_hessian_at_q(iq, tol=1e-12, use_mode_symmetry=False)
+        err = np.abs(Hall[iq] - Hd).max() / np.abs(Hd).max()
+        assert err < 1e-8, (iq, err)
+    ck = str(tmp_path / "hess")
+    h2 = QSpaceHessian.from_qspace_lanczos(h.qlanc, verbose=False)
+    h2.H_q_dict = {h.irr_qpoints[0]: Hall[h.irr_qpoints[0]]}
+    h2.save_status(ck)
+    h3 = QSpaceHessian.from_qspace_lanczos(h.qlanc, verbose=False)
+    H3 = h3.compute_full_hessian(tol=1e-12, use_mode_symmetry=False, checkpoint=ck)
+    assert np.abs(H3 - Hall).max() < 1e-10 * np.abs(Hall).max()
+    h.close()

@@ -216,3 +216,76 @@ def test_kpm_moments_harmonic_closed_form_and_doubling_identity():
     for m in range(6):
         for n in range(m + 1):
             assert abs(mu[m + n] + mu[m - n] - 2 * dot(T[m], T[n])) < 1e-10 * max(1, abs(mu[m + n]))
+
+
+def _dense(f, n):
+    M = np.zeros((n, n), complex)
+    for j in range(n):
+        e = np.zeros(n, complex)
+        e[j] = 1.0
+        M[:, j] = f(e)
+    return M
+
+
+def test_static_liouvillian_pins():
+    """Oracle static L of the Hessian (static_apply):
+      * harmonic: w^2 on R, 1 / Lambda on W; Lambda at T -> 0 is
+        1 / (4 w1 w2 (w1 + w2)), and its degenerate branch (-beta e^{beta w}
+        n^2 = dn/dw) is the limit of (n1 - n2) / (w1 - w2);
+      * the full static L is Hermitian in its metric (a wrong factor or sign
+        in the R <-> W couplings -f / -2 Y_w Y_w W breaks this);
+      * its inverse on the R sector is minus that of the spectral L (pinned to
+        the real-space reference) up to the pairs of equal frequencies, which
+        the static (isothermal) propagator keeps: within 1%."""
+    ens, modes, Xq, Yq = _system(L=2, N=40)
+    iq = 3
+    qs = _qsys(modes, Xq, Yq, ens, iq, ignore_v3=True, ignore_v4=True)
+    nb, n_s = qs.nb, qs.off_b[0]
+    rng = np.random.default_rng(6)
+    psi = rng.normal(size=n_s) + 1j * rng.normal(size=n_s)
+    out = QO.static_apply(qs, psi)
+    assert np.allclose(out[:nb], qs.w_q[:, iq] ** 2 * psi[:nb], rtol=1e-14)
+    for p in range(len(qs.unique_pairs)):
+        lam = QO.static_lambda(qs, p)
+        W = qs.get_block(psi, p, "a")
+        Mo = qs.get_block(out, p, "a")
+        nz = np.abs(lam) > 0
+        assert np.allclose(Mo[nz] * lam[nz], W[nz], rtol=1e-12)
+    # Lambda limits
+    cold = _qsys(modes, Xq, Yq, ens, iq)
+    cold.T = 0.0
+    a, b = cold.unique_pairs[0]
+    w1, w2 = cold.w_q[:, a], cold.w_q[:, b]
+    vm = np.outer(cold.valid[:, a], cold.valid[:, b])
+    ref = np.where(vm, 1.0 / (4 * np.outer(w1, w2) * np.add.outer(w1, w2) + ~vm), 0.0)
+    assert np.allclose(QO.static_lambda(cold, 0), ref, rtol=1e-14)
+    hot = _qsys(modes, Xq, Yq, ens, iq)
+    pv = next(p for p, (a, b) in enumerate(hot.unique_pairs)
+              if hot.valid[:2, a].all() and hot.valid[:2, b].all())
+    w0 = hot.w_q.max()
+    hot.w_q = hot.w_q.copy()
+    lam_eq = None
+    for d in (1e-9 * w0, 1e-4 * w0):
+        hot.w_q[:, :] = w0
+        hot.w_q[1, :] = w0 + d             # bands 0 and 1 of every q: w and w + d
+        lam = QO.static_lambda(hot, pv)
+        if lam_eq is None:
+            lam_eq = lam[0, 0]             # |d| < 1e-8: the degenerate branch
+        else:
+            assert abs(lam[0, 1] / lam_eq - 1) < 1e-3
+    # Hermitian in the metric, with the anharmonic terms
+    qs = _qsys(modes, Xq, Yq, ens, iq)
+    Ls = _dense(lambda v: QO.static_apply(qs, v), n_s)
+    m = QO.static_mask(qs)
+    MLs = m[:, None] * Ls
+    keep = m > 0
+    H = MLs[np.ix_(keep, keep)]
+    assert np.abs(H - H.conj().T).max() < 1e-9 * np.abs(H).max()
+    # against the spectral L
+    n_f = qs.psi_size
+    Lf = _dense(qs.apply_full_L, n_f)
+    kf = (np.abs(Lf).sum(0) > 0) & (np.abs(Lf).sum(1) > 0)
+    ks = (np.abs(Ls).sum(0) > 0) & (np.abs(Ls).sum(1) > 0)
+    Gf = np.linalg.inv(Lf[np.ix_(kf, kf)])[:nb, :nb]
+    Gs = np.linalg.inv(Ls[np.ix_(ks, ks)])[:nb, :nb]
+    assert np.abs(Gs + Gf).max() < 1e-2 * np.abs(Gs).max()
