@@ -275,6 +275,8 @@ def main():
 
     results = {}
     precs = [args.precision] + ([] if (args.no_fp64 or args.precision == "fp64") else ["fp64"])
+    if args.precision == "fp16x2" and not args.no_fp64:
+        precs.append("bf16x2")               # the other split variant, reported beside
     for prec in precs:
         if W > 0:
             timed_run(prec, W)
@@ -289,17 +291,28 @@ def main():
     ms_per_step = head["wall"] / K * 1e3
 
     # spectrum error of the headline path against the FP64 path
-    spec_err = None
-    if "fp64" in results and args.precision != "fp64":
+    def spec_error(chains):
         w_grid = np.linspace(0.0, 600.0, 2000) / RY_TO_CM
         eta = 5.0 / RY_TO_CM
         errs = []
-        for (a1, b1), (a0, b0) in zip(head["chains"], results["fp64"]["chains"]):
+        for (a1, b1), (a0, b0) in zip(chains, results["fp64"]["chains"]):
             s1 = spectrum(a1, b1, w_grid, eta)
             s0 = spectrum(a0, b0, w_grid, eta)
             errs.append(float(np.max(np.abs(s1 - s0)) / np.max(np.abs(s0))))
-        spec_err = max(errs)
+        return max(errs), errs
+
+    spec_err = bf16 = None
+    if "fp64" in results and args.precision != "fp64":
+        spec_err, errs = spec_error(head["chains"])
         log("spectrum L-inf relative error vs FP64: %s" % errs)
+    if "bf16x2" in results and args.precision != "bf16x2":
+        r = results["bf16x2"]
+        be, errs = spec_error(r["chains"])
+        log("bf16x2 spectrum L-inf relative error vs FP64: %s" % errs)
+        bf16 = {"value": r["n_l"] / r["wall"], "unit": "L-applications/s",
+                "speedup_vs_fp64": (r["n_l"] / r["wall"]) / (results["fp64"]["n_l"] /
+                                                            results["fp64"]["wall"]),
+                "spectrum_linf_rel_err": be}
 
     # per-kernel durations with HIP events on the library stream
     L = _lib.lib()
@@ -364,6 +377,7 @@ def main():
         "speedup_vs_fp64": (value / (results["fp64"]["n_l"] / results["fp64"]["wall"]))
         if ("fp64" in results and args.precision != "fp64") else None,
         "spectrum_linf_rel_err": spec_err,
+        "bf16x2": bf16,
         "kernel_ms": {p: [float(x) for x in prof[p]] for p in prof},
         "roofline": roof,
         "cpu_baseline": cpu,
