@@ -71,6 +71,15 @@ class QSpaceEngine:
                    "tdscha_qspace_psi_size")
         return n.value
 
+    def _check_lengths(self, pms, psis):
+        if len(psis) != len(pms):
+            raise ValueError("one pair map and one psi per chain")
+        for i, p in enumerate(psis):
+            n = self.psi_size(pms[i])
+            if len(p) != n:
+                raise ValueError("chain %d: psi of %d entries, its layout has %d"
+                                 % (i, len(p), n))
+
     def static_psi_size(self, pm):
         n = C.c_int()
         pm = np.ascontiguousarray(pm, np.int32)
@@ -84,6 +93,11 @@ class QSpaceEngine:
         """Static Liouvillian of the Hessian on psis in the static layout."""
         iqs = np.ascontiguousarray(iq_pert, np.int32)
         pms = np.ascontiguousarray(pair_maps, np.int32)
+        for i, p in enumerate(psis):
+            n = self.static_psi_size(pms[i])
+            if len(p) != n:
+                raise ValueError("chain %d: psi of %d entries, the static layout has %d"
+                                 % (i, len(p), n))
         flat = np.ascontiguousarray(np.concatenate(psis), np.complex128)
         out = np.zeros_like(flat)
         _lib.check(_lib.lib().tdscha_qspace_apply_L_static(
@@ -100,6 +114,7 @@ class QSpaceEngine:
                 ignore_v4=False):
         iqs = np.ascontiguousarray(iq_pert, np.int32)
         pms = np.ascontiguousarray(pair_maps, np.int32)
+        self._check_lengths(pms, psis)
         flat = np.ascontiguousarray(np.concatenate(psis), np.complex128)
         out = np.zeros_like(flat)
         _lib.check(_lib.lib().tdscha_qspace_apply_L(
@@ -117,6 +132,7 @@ class QSpaceEngine:
         iqs = np.ascontiguousarray(iq_pert, np.int32)
         pms = np.ascontiguousarray(pair_maps, np.int32)
         nc = len(iqs)
+        self._check_lengths(pms, psis)
         flat = np.ascontiguousarray(np.concatenate(psis), np.complex128)
         a = np.zeros((nc, n_steps))
         b = np.zeros((nc, n_steps))
@@ -144,6 +160,16 @@ class QSpaceEngine:
         pms = np.ascontiguousarray(pair_maps, np.int32)
         nc = len(iqs)
         sizes = [len(x) for x in vs]
+        if not (len(vms) == len(vs) == len(pms) == nc):
+            raise ValueError("one pair map, vm and v per chain")
+        for i in range(nc):
+            n = self.psi_size(pms[i])
+            if len(vms[i]) != n or len(vs[i]) != n:
+                raise ValueError("chain %d: vectors of %d / %d entries, its layout has %d"
+                                 % (i, len(vms[i]), len(vs[i]), n))
+        if n_have and (moments is None or np.shape(moments)[0] != nc
+                       or np.shape(moments)[1] < n_have):
+            raise ValueError("continuing needs the first n_have moments of every chain")
         fvm = np.ascontiguousarray(np.concatenate(vms), np.complex128)
         fv = np.ascontiguousarray(np.concatenate(vs), np.complex128)
         res = np.ascontiguousarray(rescale, np.float64).reshape(nc, 2)

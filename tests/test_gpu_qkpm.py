@@ -148,3 +148,26 @@ def test_batched_equals_single_runs_graph_and_plain():
         assert abs(kpm.kpm_vector_norm - norms[i]) < 1e-15 * norms[i]
     print("batched KPM: 4 chains x 24 moments in %.2f ms" % ms)
     kpm.close()
+
+
+def test_argument_errors_are_reported():
+    """Wrong vector lengths and bad continuation arguments raise before any
+    device work; the library rejects n_have >= n_moments."""
+    from tdscha_amd import _lib
+    ens, kpm, band = _kpm()
+    pm = kpm.q_pair_map
+    v = kpm.psi / np.sqrt(kpm._metric_dot(kpm.psi, kpm.psi))
+    eng = kpm.engine
+    with pytest.raises(ValueError):
+        eng.kpm([0], [pm], [v[:-1]], [v[:-1]], [(1.0, 0.0)], 8)
+    with pytest.raises(ValueError):
+        eng.kpm([0], [pm], [v], [v], [(1.0, 0.0)], 8, n_have=4, moments=None)
+    with pytest.raises(_lib.TdschaError, match="n_have"):
+        eng.kpm([0], [pm], [v], [v], [(1.0, 0.0)], 8, n_have=8, moments=np.ones((1, 8)))
+    with pytest.raises(_lib.TdschaError, match="rescale"):
+        eng.kpm([0], [pm], [v], [v], [(0.0, 0.0)], 8)
+    with pytest.raises(ValueError):
+        eng.apply_L_static([0], [pm], [v])            # spectral layout into the static operator
+    with pytest.raises(ValueError):
+        eng.apply_L([0], [pm], [v[:5]])
+    kpm.close()
