@@ -394,3 +394,31 @@ def static_mask(qs):
             np.fill_diagonal(full, np.where(qs.valid[:, iq1] & qs.valid[:, iq2], 1.0, 0.0))
             m[off:off + qs.block_sizes[p]] = full[np.triu_indices(nb)]
     return m
+
+
+def kpm_moments_doubling(matvec, psi0, mask, n_moments, rescale_a, rescale_b):
+    """The moments as QSpaceKPM.run_KPM computes them (QSK:301-440, fresh
+    start): two per L application through mu_{2n} = 2 <t_n, t_n>_M - mu_0 and
+    mu_{2n+1} = 2 <t_{n+1}, t_n>_M - mu_1.  Equal to kpm_moments_direct when L
+    is self-adjoint in the metric (the 2x2x2 grid); the reference's q-space L
+    on odd grids is not, and there the two differ."""
+    dot = lambda u, v: float(np.real(np.conj(u).dot(v * mask)))
+    norm = np.sqrt(dot(psi0, psi0))
+    v0 = psi0 / norm
+    Lt = lambda v: (matvec(v) - rescale_b * v) / rescale_a
+    mu = np.zeros(n_moments)
+    mu[0] = 1.0
+    if n_moments == 1:
+        return mu, norm
+    v1 = Lt(v0)
+    mu[1] = dot(v0, v1)
+    vm, v, n = v0, v1, 1
+    while 2 * n < n_moments:
+        mu[2 * n] = 2 * dot(v, v) - mu[0]
+        if 2 * n + 1 >= n_moments:
+            break
+        vp = 2 * Lt(v) - vm
+        mu[2 * n + 1] = 2 * dot(vp, v) - mu[1]
+        vm, v = v, vp
+        n += 1
+    return mu, norm

@@ -26,15 +26,17 @@ def _oracle(h, iq, **kw):
     return QO.QSystem(ql.X_q, ql.Y_q, ql.w_q, ql.valid_modes_q, ql.rho, ql.T, pm, pairs, iq, **kw)
 
 
-@pytest.mark.parametrize("flags", [(0, 0, 0), (0, 1, 0), (0, 0, 1), (1, 0, 0)])
-def test_static_L_matches_oracle(flags):
+@pytest.mark.parametrize("L,flags", [(2, (0, 0, 0)), (2, (0, 1, 0)), (2, (0, 0, 1)),
+                                     (2, (1, 0, 0)), (3, (0, 0, 0))])
+def test_static_L_matches_oracle(L, flags):
     from tdscha_amd.qhessian import QSpaceHessian
     ih, i3, i4 = flags
-    h = QSpaceHessian(_system(), verbose=False, ignore_v3=bool(i3), ignore_v4=bool(i4))
+    h = QSpaceHessian(_system(L=L, N=40 if L == 2 else 20), verbose=False, ignore_v3=bool(i3),
+                      ignore_v4=bool(i4))
     h.ignore_harmonic = bool(ih)
     h.init(use_symmetries=False)
     rng = np.random.default_rng(8)
-    for iq in (0, 3, 6):
+    for iq in ((0, 3, 6) if L == 2 else (0, 4, 13)):
         h._layout(iq)
         qs = _oracle(h, iq, ignore_harmonic=bool(ih), ignore_v3=bool(i3), ignore_v4=bool(i4))
         psi = rng.normal(size=h._size) + 1j * rng.normal(size=h._size)

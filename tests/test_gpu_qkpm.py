@@ -25,18 +25,29 @@ def _kpm(L=2, N=60, ignore_v3=False, ignore_v4=False, sym=False, seed=5):
     return ens, kpm, band
 
 
-def test_moments_match_the_oracle_recurrence():
-    ens, kpm, band = _kpm()
+@pytest.mark.parametrize("L,iq", [(2, 0), (2, 3), (3, 0), (3, 5)])
+def test_moments_match_the_oracle_recurrence(L, iq):
+    """Against the oracle restatement of the reference's two-moments-per-L
+    run on 2x2x2 and 3x3x3 grids, at Gamma and at q_pert != Gamma; on the
+    2x2x2 grid, where L is self-adjoint in the metric, also against the
+    one-moment-per-L recurrence.  (On odd grids the reference's q-space L is
+    not self-adjoint in its metric, so there the two-moment identity, and
+    with it the reference's KPM, departs from the direct moments.)"""
+    ens, kpm, band = _kpm(L=L, N=60 if L == 2 else 30)
+    kpm.prepare_mode_q(iq, int(np.argmax(kpm.w_q[:, iq])))
     kpm.run_KPM(33, verbose=False)
     from tdscha_amd.qspace import pair_map
-    pm, pairs = pair_map(kpm.L, 0)
-    qs = QO.QSystem(kpm.X_q, kpm.Y_q, kpm.w_q, kpm.valid_modes_q, ens.rho, ens.T, pm, pairs, 0)
-    mu, norm = QO.kpm_moments_direct(qs.apply_full_L, kpm.psi, qs.mask(), 33, kpm.kpm_rescale_a,
-                                     kpm.kpm_rescale_b)
+    pm, pairs = pair_map(kpm.L, iq)
+    qs = QO.QSystem(kpm.X_q, kpm.Y_q, kpm.w_q, kpm.valid_modes_q, ens.rho, ens.T, pm, pairs, iq)
+    args = (qs.apply_full_L, kpm.psi, qs.mask(), 33, kpm.kpm_rescale_a, kpm.kpm_rescale_b)
+    mu, norm = QO.kpm_moments_doubling(*args)
     err = np.abs(kpm.kpm_moments - mu).max()
-    print("moments vs oracle: max abs diff %.2e" % err)
+    print("L=%d iq=%d moments vs oracle: max abs diff %.2e" % (L, iq, err))
     assert abs(kpm.kpm_vector_norm - norm) < 1e-14
     assert err < 1e-11
+    if L == 2:
+        mu_d, _ = QO.kpm_moments_direct(*args)
+        assert np.abs(kpm.kpm_moments - mu_d).max() < 1e-11
     kpm.close()
 
 

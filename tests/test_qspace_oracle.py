@@ -216,6 +216,9 @@ def test_kpm_moments_harmonic_closed_form_and_doubling_identity():
     for m in range(6):
         for n in range(m + 1):
             assert abs(mu[m + n] + mu[m - n] - 2 * dot(T[m], T[n])) < 1e-10 * max(1, abs(mu[m + n]))
+    # the two-moments-per-L restatement equals the direct one here
+    mu2, _ = QO.kpm_moments_doubling(qs.apply_full_L, psi, mask, 12, a, b)
+    assert np.abs(mu2 - mu).max() < 1e-10
 
 
 def _dense(f, n):
