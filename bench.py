@@ -191,14 +191,12 @@ def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True, tiles
     # kernel polynomial method on the same batch: 256 Chebyshev moments per
     # chain (128 L applications), one graph launch
     from tdscha_amd.qkpm import kpm_bounds
-    from tdscha_amd.qlanczos import QSpaceLanczos
+    from tdscha_amd.qlanczos import wigner_mask
     lmin, lmax = kpm_bounds(modes.w_q, modes.valid_modes_q, True, bound_factor=1.2)
-    ql = QSpaceLanczos()
-    ql.n_bands, ql.L = nb, modes.L
     v0s = []
     for iq, psi in zip(iqs, psis):
-        ql.build_q_pair_map(int(iq))
-        v0s.append(psi / np.sqrt(np.real(np.conj(psi) @ (psi * ql.mask_dot_wigner()))))
+        mask = wigner_mask(nb, pair_map(modes.L, int(iq))[1])
+        v0s.append(psi / np.sqrt(np.real(np.conj(psi) @ (psi * mask))))
     n_mom = 256
     t = time.perf_counter()
     _, _, _, kpm_ms = eng.kpm(iqs, pms, v0s, v0s, [(0.5 * (lmax - lmin), 0.5 * (lmax + lmin))] * nq,

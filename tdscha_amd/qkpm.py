@@ -17,7 +17,7 @@ import os
 
 import numpy as np
 
-from .qlanczos import EPSILON, QSpaceLanczos
+from .qlanczos import EPSILON, QSpaceLanczos, wigner_mask
 
 RY_TO_CM = 109737.31568527
 
@@ -188,11 +188,7 @@ class QSpaceKPM(QSpaceLanczos):
         pms, v0s, norms = [], [], []
         for iq, psi in zip(iq_perts, psis):
             pm, pairs = pair_map(self.L, int(iq))
-            nb = self.n_bands
-            mask = np.concatenate([np.ones(nb)] + 2 * [
-                np.full(nb * nb, 2.0) if p < q else
-                np.where(np.triu_indices(nb)[0] == np.triu_indices(nb)[1], 1.0, 2.0)
-                for p, q in pairs])
+            mask = wigner_mask(self.n_bands, pairs)
             psi = np.asarray(psi, np.complex128)
             if len(psi) != len(mask):
                 raise ValueError("psi of q-point %d has %d entries, its layout %d"

@@ -228,6 +228,16 @@ class QSpaceEngine:
             pass
 
 
+def wigner_mask(n_bands, unique_pairs):
+    """Metric of the psi layout (QL:441-480): 1 on R, 2 on full pair blocks,
+    1 / 2 on the diagonal / off-diagonal of the upper triangles."""
+    nb = n_bands
+    iu = np.triu_indices(nb)
+    tri = np.where(iu[0] == iu[1], 1.0, 2.0)
+    blocks = [np.full(nb * nb, 2.0) if a < b else tri for a, b in unique_pairs]
+    return np.concatenate([np.ones(nb)] + blocks + blocks)
+
+
 class QSpaceLanczos(object):
     """q-space Lanczos of a rock-salt ensemble (QSpaceLanczos, QL:116)."""
 
@@ -302,12 +312,7 @@ class QSpaceLanczos(object):
 
     def mask_dot_wigner(self, debug=False):
         """QL:441-480."""
-        nb = self.n_bands
-        m = [np.ones(nb)]
-        iu = np.triu_indices(nb)
-        tri = np.where(iu[0] == iu[1], 1.0, 2.0)
-        blocks = [np.full(nb * nb, 2.0) if a < b else tri for a, b in self.unique_pairs]
-        return np.concatenate(m + blocks + blocks)
+        return wigner_mask(self.n_bands, self.unique_pairs)
 
     def reset_q(self):
         self.psi = np.zeros(self.get_psi_size(), np.complex128)
