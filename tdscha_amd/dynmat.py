@@ -267,3 +267,28 @@ def load_ensemble_bin(directory, population):
     if x.shape != f.shape or x.ndim != 3 or x.shape[2] != 3:
         raise ValueError("positions and forces must both be (N, nat, 3)")
     return x, f
+
+
+class SupercellModel:
+    """The supercell of a DynSet in the form the Bloch-mode builder and the
+    q-space symmetries read (qspace.BlochModes, qspace.q_symmetries): cubic
+    L x L x L supercell, per-coordinate masses ``m``, force constants, the
+    primitive cell, species, and positions taken about the first atom of the
+    unit cell (the point group acts about that site)."""
+
+    def __init__(self, dyn):
+        dims = dyn.check_grid()
+        if len(set(dims)) != 1:
+            raise ValueError("q space needs an L x L x L supercell, the files give %s" % (dims,))
+        self.L = dims[0]
+        pos, lat, spec, mass = supercell_structure(dyn, dims)
+        self.cell = np.asarray(dyn.cell, float)
+        self.lattice = lat
+        self.positions = pos - dyn.tau[0]
+        self.species = spec
+        self.nat = len(pos)
+        self.m = np.repeat(mass, 3)
+        self._phi = supercell_force_constants(dyn, pos)
+
+    def force_constants(self):
+        return self._phi

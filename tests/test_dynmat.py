@@ -112,3 +112,31 @@ def test_reader_errors(tmp_path, dyn):
     (tmp_path / "dyn1").write_text("\n".join(lines) + "\n")
     with pytest.raises(NotImplementedError):
         DM.read_dyn(str(tmp_path / "dyn1"))
+
+
+def test_supercell_model_bloch_modes_and_point_group(dyn):
+    """The files as a q-space model: the Bloch basis diagonalises the
+    supercell dynamical matrix, its frequencies are those of the supercell
+    modes (and at Gamma the printed ones), and the 48 operations of O_h map
+    the supercell onto itself."""
+    from tdscha_amd.qspace import BlochModes, q_symmetries
+    sm = DM.SupercellModel(dyn)
+    assert sm.L == 2 and sm.nat == 16
+    bm = BlochModes(sm)
+    E = bm.basis()
+    assert np.abs(E.conj().T @ E - np.eye(48)).max() < 1e-13
+    s = 1 / np.sqrt(sm.m)
+    M = E.conj().T @ (sm.force_constants() * s[:, None] * s[None, :]) @ E
+    w2 = (np.sign(bm.w_q) * bm.w_q ** 2).T.ravel()
+    assert np.abs(M - np.diag(w2)).max() < 1e-13 * np.abs(w2).max()
+    pos, _, _, mass = DM.supercell_structure(dyn)
+    w, _ = DM.supercell_modes(sm.force_constants(), mass)
+    assert np.abs(np.sort(bm.w_q.ravel())[3:] - w[3:]).max() < 1e-13 * w.max()
+    g = DM.q_frequencies(dyn.files[0])
+    assert np.abs(np.sort(bm.w_q[:, 0]) * DM.RY_TO_CM / g - 1)[3:].max() < 1e-12
+    perm, D = q_symmetries(bm)
+    assert perm.shape == (48, 8) and D.shape == (48, 8, 6, 6)
+    for k in range(48):
+        assert sorted(perm[k]) == list(range(8))
+        for iq in range(8):
+            assert np.abs(D[k, iq] @ D[k, iq].conj().T - np.eye(6)).max() < 1e-12
