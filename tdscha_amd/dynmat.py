@@ -280,6 +280,9 @@ class SupercellModel:
         dims = dyn.check_grid()
         if len(set(dims)) != 1:
             raise ValueError("q space needs an L x L x L supercell, the files give %s" % (dims,))
+        if dyn.nat != 2:
+            raise NotImplementedError("the Bloch-mode builder has six bands: two atoms per cell, "
+                                      "the files have %d" % dyn.nat)
         self.L = dims[0]
         pos, lat, spec, mass = supercell_structure(dyn, dims)
         self.cell = np.asarray(dyn.cell, float)

@@ -112,6 +112,9 @@ def test_reader_errors(tmp_path, dyn):
     (tmp_path / "dyn1").write_text("\n".join(lines) + "\n")
     with pytest.raises(NotImplementedError):
         DM.read_dyn(str(tmp_path / "dyn1"))
+    # q space: the incomplete grid is refused before any Bloch transform
+    with pytest.raises(ValueError, match="distinct"):
+        DM.SupercellModel(DM.DynSet(dyn.files[:2]))
 
 
 def test_supercell_model_bloch_modes_and_point_group(dyn):
