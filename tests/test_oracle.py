@@ -195,3 +195,29 @@ def test_continued_fraction_closed_forms():
     Tm = np.diag(a3) + np.diag(b3[:2], 1) + np.diag(b3[:2], -1)
     z = wz[0] ** 2 - 2j * wz[0] * 0.05
     assert np.allclose(g[0], np.linalg.inv(Tm - z * np.eye(3))[0, 0], rtol=1e-12)
+
+
+def test_wigner_L_symmetry_and_the_translational_projector():
+    """The Wigner L of the reference is symmetric in the mask metric
+    (DL:5905-5954) with the full space group as images, and stays so for the
+    harmonic + D4 part under the gamma-only translational projector; its D3
+    term is not, since that projector acts on the average force only
+    (DL:2951-3363).  The full-size GPU test relies on this split."""
+    from tdscha_amd.rocksalt import Ensemble, RockSalt
+    ens = Ensemble(RockSalt(L=2), 300.0, 100, seed=0)
+    n = len(ens.w)
+    m = O.mask_dot_wigner(n)
+    x, y = np.random.default_rng(11).standard_normal((2, n + n * (n + 1)))
+
+    def asym(gamma, **flags):
+        ms = ens.symmetries(use_symmetries=True, gamma_only=gamma)
+        kw = dict(pols=ens.pols, trans_projector=ms.trans_projector,
+                  inv_perms=ms.trans_cart_perms) if gamma else {}
+        orc = O.OracleSystem(ens.X, ens.Y, ens.w, ens.rho, ens.T, ms.dense(), use_wigner=True,
+                             **kw, **flags)
+        Lx, Ly = orc.apply_full_L(x.copy()), orc.apply_full_L(y.copy())
+        return abs(np.dot(x * m, Ly) - np.dot(Lx * m, y)) / np.sqrt(np.dot(x * m, x) * np.dot(Ly * m, Ly))
+
+    assert asym(False) < 1e-15
+    assert asym(True, ignore_v3=True) < 1e-15
+    assert asym(True) > 1e-7
