@@ -257,7 +257,7 @@ def main():
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the ensemble sum "
               "would shard by configuration with one all-reduce of n + n^2 doubles per L "
-              "application; see DESIGN.md).  Run with --gpus 1.", file=sys.stderr)
+              "application).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from tdscha_amd import _lib

@@ -1,7 +1,7 @@
 """Generate Lanczos chains on the GPU and train the learned terminator.
 
     python -m tdscha_amd.train_terminator --out-dir DIR [--n-train 5000]
-        [--n-heldout 200] [--train-steps 400] [--judge-steps 1000]
+        [--n-heldout 200] [--train-steps 400] [--ref-steps 1000]
 
 Training data: rock-salt 2x2x2 spring models (16 atoms, 45 modes) with
 k3 ~ U(-4, 0) eV/A^3, k4 ~ U(2, 10) eV/A^4, T ~ U(50, 600) K, one seeded
@@ -10,7 +10,7 @@ three chains (IR-like x, y, z perturbations, Wigner, full
 reorthogonalisation, FP64) run as one batch by the library's device Lanczos.
 Ensembles are built by CPU worker processes started before the GPU is
 touched.  Held-out potentials come from a separate seed stream and are run
-to ``--judge-steps``; their spectra judge the standard and the learned
+to ``--ref-steps``; their spectra are the reference for the standard and the learned
 terminator at K steps.  Writes DIR/terminator_rocksalt16.npz (the trained
 ensemble) and DIR/terminator_rocksalt16.json (held-out report).
 """
@@ -94,7 +94,7 @@ def spectrum(a, b, w, eta, last_average=1):
 
 
 def evaluate(term, A, B, lens, w, eta, K, k_max):
-    """Held-out report: L-inf spectrum errors (relative to the judge's maximum)
+    """Held-out report: L-inf spectrum errors (relative to the long chain's maximum)
     of the standard terminator at K, of the learned terminator at K, and the
     number of steps the standard terminator needs to match the learned one."""
     rows = []
@@ -144,7 +144,7 @@ def main():
     ap.add_argument("--n-train", type=int, default=5000)
     ap.add_argument("--n-heldout", type=int, default=200)
     ap.add_argument("--train-steps", type=int, default=400)
-    ap.add_argument("--judge-steps", type=int, default=1000)
+    ap.add_argument("--ref-steps", type=int, default=1000)
     ap.add_argument("--configs", type=int, default=2000)
     ap.add_argument("--K", type=int, default=40)
     ap.add_argument("--members", type=int, default=4)
@@ -166,7 +166,7 @@ def main():
     ptrain = potential_params(n_pot, args.seed)
     A, B, lens = run_chains(pool, ptrain, 100000, args.configs, args.train_steps, "train")
     pheld = potential_params(args.n_heldout, args.seed + 7919)
-    Ah, Bh, lh = run_chains(pool, pheld, 900000, args.configs, args.judge_steps, "held-out")
+    Ah, Bh, lh = run_chains(pool, pheld, 900000, args.configs, args.ref_steps, "held-out")
     pool.close()
     pool.join()
     t_gen = time.time() - t0
@@ -192,7 +192,7 @@ def main():
     rep = evaluate(term, Ah, Bh, lh, w, eta, args.K, 120)
     rep.update({
         "K": args.K, "n_train_chains": int(len(a_tr)), "train_steps": args.train_steps,
-        "judge_steps": args.judge_steps, "n_heldout_potentials": args.n_heldout,
+        "ref_steps": args.ref_steps, "n_heldout_potentials": args.n_heldout,
         "configs_per_ensemble": args.configs, "members": args.members,
         "generation_s": t_gen, "training_s": t_train, "var_threshold": term.var_threshold,
         "grid": "2000 points 0-600 cm^-1, smearing 5 cm^-1",

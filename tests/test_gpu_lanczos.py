@@ -128,7 +128,7 @@ def _rocksalt_lanczos(N=2000, seed=0, use_symmetries=False, gamma=False, L=2):
 
 
 def test_device_lanczos_full_reorth_matches_oracle():
-    """configs[0] workload: 16-atom rock salt, N=2000, no symmetries, IR x
+    """16-atom rock salt workload, N=2000, no symmetries, IR x
     perturbation, Wigner; 50 steps with full reorthogonalisation.  Gate:
     a_n, b_n within 1e-10 relative of the FP64 oracle."""
     from oracle import lanczos_oracle as O

@@ -36,7 +36,7 @@ def test_snte_full_group(snte, apply_d4):
 
 
 @pytest.mark.parametrize("n,N,blocks,nsym", [
-    (45, 2000, [1] * 45, 1),                       # configs[0] shape, no symmetry
+    (45, 2000, [1] * 45, 1),                       # 16 atoms, N = 2000, no symmetry
     (130, 300, [3, 5, 2] + [1] * 120, 6),          # two column tiles + padding
     (256, 97, [8, 8] + [4] * 60, 3),               # exact tile multiple, odd N
 ])

@@ -182,7 +182,7 @@ def test_spectral_training_reduces_loss():
 def test_learned_terminator_gains_over_standard_on_heldout_potentials():
     """Held-out potentials (seeds outside the training streams): spectra of
     40-step chains closed by the learned tail versus the standard constant
-    terminator, judged against 1000-step FP64 chains of the same potential."""
+    terminator, scored against 1000-step FP64 chains of the same potential."""
     from tdscha_amd.lanczos import Lanczos, continued_fraction
     from tdscha_amd.rocksalt import Ensemble, RockSalt
     from tdscha_amd.terminator import LearnedTerminator
