@@ -255,7 +255,7 @@ def main():
                     help="bytes of one device-to-device copy to run first (counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the ensemble sum "
+        print("bench.py measures one GPU: the library has no multi-GPU path (the ensemble sum "
               "would shard by configuration with one all-reduce of n + n^2 doubles per L "
               "application).  Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)

@@ -1,4 +1,4 @@
-"""ctypes access to the two C checkers.  TEST INFRASTRUCTURE ONLY.
+"""ctypes access to the two C oracles.  TEST INFRASTRUCTURE ONLY.
 
 * ``libtdscha_oracle.so``  - the project's C restatement (oracle/perturb_avg.c)
 * ``_ref/libref_lanczos.so`` - the reference's own LanczosFunctions.c compiled

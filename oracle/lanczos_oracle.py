@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` may import this module, and only as the
-checker.  The product package ``tdscha_amd`` never imports it.
+reference for comparison.  The product package ``tdscha_amd`` never imports it.
 
 Each function restates one piece of the reference algorithm (SSCHAcode/tdscha)
 in plain numpy; the docstring of every function cites the reference file:line

@@ -1,7 +1,7 @@
 /*
  * CPU oracle: C restatement of the ensemble-averaged D3/D4 application.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the CPU baseline port); the
+ * TEST INFRASTRUCTURE ONLY (the test oracle and the CPU baseline port); the
  * product library never links it.
  *
  * Follows the reference per-image loop: for every configuration i and
