@@ -3,7 +3,7 @@ Lanczos built from the dynamical matrices (Bloch modes of the files, Bloch
 transform of the prepared ensemble, 48 point-group images) against the
 real-space Lanczos of the same prepared ensemble with its detected 384
 operations.  On the 2x2x2 grid every q equals -q, the two operators are the
-same, so the chains agree to rounding (the reference's own comparison,
+same, so the chains agree to rounding at each of the 8 q-points (the reference's own comparison,
 tests/test_qspace/test_qspace_lanczos.py:154-270, allows 1% on the Green
 function)."""
 import os
@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 RAW = os.path.join(GOLDEN, "snte_raw")
 
 
-@pytest.mark.parametrize("iq", [0, 1])
+@pytest.mark.parametrize("iq", range(8))
 def test_qspace_from_files_equals_real_space(iq):
     from tdscha_amd import dynmat as DM
     from tdscha_amd import preprocess as PP
