@@ -239,6 +239,9 @@ def main():
     ap.add_argument("--precision", default="fp16x2")
     ap.add_argument("--supercell", type=int, default=4)
     ap.add_argument("--configs", type=int, default=20000)
+    ap.add_argument("--gate-steps", type=int, default=200,
+                    help="chain length of the split-vs-FP64 accuracy figures (untimed runs "
+                         "when --steps is shorter)")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -310,28 +313,69 @@ def main():
     ms_per_step = head["wall"] / K * 1e3
 
     # spectrum error of the headline path against the FP64 path
-    def spec_error(chains):
+    def spec_error(chains, ref_chains=None):
         w_grid = np.linspace(0.0, 600.0, 2000) / RY_TO_CM
         eta = 5.0 / RY_TO_CM
         errs = []
-        for (a1, b1), (a0, b0) in zip(chains, results["fp64"]["chains"]):
+        ref_chains = results["fp64"]["chains"] if ref_chains is None else ref_chains
+        for (a1, b1), (a0, b0) in zip(chains, ref_chains):
             s1 = spectrum(a1, b1, w_grid, eta)
             s0 = spectrum(a0, b0, w_grid, eta)
             errs.append(float(np.max(np.abs(s1 - s0)) / np.max(np.abs(s0))))
         return max(errs), errs
 
+    # Accuracy of the split paths against FP64 on the configured chains: 200
+    # steps per chain whatever --steps is (untimed runs when the timed ones
+    # were shorter), the spectrum on the 2000-point grid, the first 50
+    # coefficients and one L application of each starting vector.
     spec_err = bf16 = None
+    acc = {}
+    gate_steps = max(K, args.gate_steps)
     if "fp64" in results and args.precision != "fp64":
-        spec_err, errs = spec_error(head["chains"])
-        log("spectrum L-inf relative error vs FP64: %s" % errs)
+        short = {p: spec_error(results[p]["chains"])[0] for p in results if p != "fp64"}
+        long_chains = {}
+        for prec in results:
+            if K >= gate_steps:
+                long_chains[prec] = results[prec]["chains"]
+            else:
+                t0 = time.perf_counter()
+                long_chains[prec] = lanc.run_FT_batched(psis, gate_steps, reorth=True,
+                                                        precision=prec)[0]
+                log("%s: %d-step chains for the accuracy gate (untimed, %.1fs)" % (
+                    prec, gate_steps, time.perf_counter() - t0))
+        lanc.precision = "fp64"
+        L64 = lanc.apply_L_batch(psis)
+        for prec in results:
+            if prec == "fp64":
+                continue
+            se, errs = spec_error(long_chains[prec], long_chains["fp64"])
+            ea = eb = 0.0
+            for (a1, b1), (a0, b0) in zip(long_chains[prec], long_chains["fp64"]):
+                k = min(50, len(a1), len(a0))
+                ea = max(ea, float(np.max(np.abs(a1[:k] - a0[:k]) / np.abs(a0[:k]))))
+                kb = min(50, len(b1), len(b0))
+                eb = max(eb, float(np.max(np.abs(b1[:kb] - b0[:kb]) / np.abs(b0[:kb]))))
+            lanc.precision = prec
+            Ls = lanc.apply_L_batch(psis)
+            lanc.precision = "fp64"
+            lv = float(np.max(np.linalg.norm(Ls - L64, axis=1) / np.linalg.norm(L64, axis=1)))
+            acc[prec] = {"spectrum_linf_rel_err": se, "chain_steps": gate_steps,
+                         "spectrum_linf_rel_err_per_chain": errs,
+                         "spectrum_linf_rel_err_timed_chains": short[prec],
+                         "timed_chain_steps": K,
+                         "coeff_a_max_rel_err_first50": ea, "coeff_b_max_rel_err_first50": eb,
+                         "Lv_rel_l2_err": lv}
+            log("%s vs FP64 on %d-step chains: spectrum %.2e (per chain %s), a %.2e, b %.2e "
+                "(first 50), L.v %.2e; %d-step timed chains: spectrum %.2e" % (
+                    prec, gate_steps, se, ["%.2e" % x for x in errs], ea, eb, lv, K,
+                    short[prec]))
+        spec_err = acc[args.precision]["spectrum_linf_rel_err"]
     if "bf16x2" in results and args.precision != "bf16x2":
         r = results["bf16x2"]
-        be, errs = spec_error(r["chains"])
-        log("bf16x2 spectrum L-inf relative error vs FP64: %s" % errs)
         bf16 = {"value": r["n_l"] / r["wall"], "unit": "L-applications/s",
                 "speedup_vs_fp64": (r["n_l"] / r["wall"]) / (results["fp64"]["n_l"] /
-                                                            results["fp64"]["wall"]),
-                "spectrum_linf_rel_err": be}
+                                                            results["fp64"]["wall"])}
+        bf16.update(acc.get("bf16x2", {}))
 
     # per-kernel durations with HIP events on the library stream
     L = _lib.lib()
@@ -397,6 +441,7 @@ def main():
         "speedup_vs_fp64": (value / (results["fp64"]["n_l"] / results["fp64"]["wall"]))
         if ("fp64" in results and args.precision != "fp64") else None,
         "spectrum_linf_rel_err": spec_err,
+        "accuracy": acc.get(args.precision),
         "bf16x2": bf16,
         "kernel_ms": {p: [float(x) for x in prof[p]] for p in prof},
         "roofline": roof,

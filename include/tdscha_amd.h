@@ -79,11 +79,30 @@ int tdscha_perturb_averages(tdscha_ctx* ctx, const double* R1, const double* Y1,
  *   precision: 0 = FP64, 1 = split FP16 hi/lo (FP32 accumulation),
  *              2 = split BF16 hi/lo (FP32 accumulation).
  *   stream: hipStream_t to run on (NULL = the context's stream).
- * Replaces get_perturb_averages_sym (Modules/tdscha_core.jl:212-238).
+ * Replaces get_perturb_averages_sym (Modules/tdscha_core.jl:212-238) over
+ * the whole index range [1, n_configs * n_syms]; its sub-ranges are
+ * tdscha_perturb_averages_range below.
  */
 int tdscha_perturb_averages_dev(tdscha_ctx* ctx, int n_chains, const double* R1_dev,
                                 const double* Y1_dev, int apply_d4, int precision, double* f_dev,
                                 double* d2v_dev, void* stream);
+
+/*
+ * Image-range variant of tdscha_perturb_averages_dev with the index
+ * semantics of get_perturb_averages_sym(..., start_index, end_index)
+ * (Modules/tdscha_core.jl:164-238, called per rank at
+ * Modules/DynamicalLanczos.py:3141-3164): only the images
+ * bigindex = (config - 1) * n_syms + sym, 1-based, start_index <= bigindex
+ * <= end_index, are summed, and the partial result is still divided by the
+ * global N_eff * n_syms, so that the partials of any split of
+ * [1, n_configs * n_syms] add up to the full call.  Returns 2 unless
+ * 1 <= start_index <= end_index <= n_configs * n_syms (the reference's
+ * @assert, JC:178-180).
+ */
+int tdscha_perturb_averages_range(tdscha_ctx* ctx, int n_chains, const double* R1_dev,
+                                  const double* Y1_dev, int apply_d4, int precision,
+                                  long start_index, long end_index, double* f_dev,
+                                  double* d2v_dev, void* stream);
 
 /* Batched host-pointer variant (same layouts as the _dev entry, synchronous). */
 int tdscha_perturb_averages_batched(tdscha_ctx* ctx, int n_chains, const double* R1,
@@ -155,8 +174,9 @@ int tdscha_continued_fraction(int n_a, const double* a, int n_b, const double* b
  *             reference layout X_q[iq, config, nu]
  *   w_q       [n_bands, n_q] frequencies (Ry); valid_modes_q [n_bands, n_q]
  *             (0 = masked acoustic band); rho [n_configs]; T (K)
- * n_bands must be 3 or 6.  Point-group images are not supported (one
- * symmetry, the identity, as QSpaceLanczos.prepare_symmetrization(no_sym=True)).
+ * n_bands must be 3 or 6.  The context starts with one symmetry, the
+ * identity (as QSpaceLanczos.prepare_symmetrization(no_sym=True));
+ * tdscha_qctx_symmetrize below adds the point-group images.
  */
 typedef struct tdscha_qctx tdscha_qctx;
 int tdscha_qctx_create(tdscha_qctx** out, int n_q, int n_bands, int n_configs, const double* X_q,

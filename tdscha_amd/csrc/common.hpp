@@ -102,6 +102,10 @@ struct Ctx {
   long Mp = 0;    // padded images (multiple of the K split granule)
   double T = 0;
   double n_eff = 0;
+  // image range of the next perturbed-average pass, [row_lo, row_hi) in
+  // image order b = i * n_syms + j (default: every image)
+  long row_lo = 0;
+  long row_hi = 0;
   // symmetric image ensemble, device resident
   double* U = nullptr;      // [Mp, np] S_j x_i
   double* F = nullptr;      // [Mp, np] S_j y_i

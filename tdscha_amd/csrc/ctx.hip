@@ -130,6 +130,8 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
   c->n_syms = n_syms;
   c->M = (long)n_configs * n_syms;
   c->Mp = round_up_l(c->M, 128);
+  c->row_lo = 0;
+  c->row_hi = c->M;
   c->T = T;
   double ne = 0;
   for (int i = 0; i < n_configs; ++i) ne += rho[i];
@@ -295,6 +297,28 @@ int tdscha_perturb_averages_dev(tdscha_ctx* ctx, int n_chains, const double* R1_
   hipLaunchKernelGGL(pad_inputs, g, dim3(256), 0, s, R1_dev, Y1_dev, c->n, c->np, c->R1, c->Y1);
   TDS_CHECK_HIP(hipGetLastError());
   return run_pa(c, n_chains, apply_d4, precision, f_dev, d2v_dev, s);
+}
+
+int tdscha_perturb_averages_range(tdscha_ctx* ctx, int n_chains, const double* R1_dev,
+                                  const double* Y1_dev, int apply_d4, int precision,
+                                  long start_index, long end_index, double* f_dev,
+                                  double* d2v_dev, void* stream) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_perturb_averages_range: NULL context");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  // the reference's assertions (Modules/tdscha_core.jl:178-180)
+  TDS_REQUIRE(start_index <= end_index, 2,
+              "tdscha_perturb_averages_range: start_index must not exceed end_index");
+  TDS_REQUIRE(start_index >= 1, 2, "tdscha_perturb_averages_range: start_index must be >= 1");
+  TDS_REQUIRE(end_index <= c->M, 2,
+              "tdscha_perturb_averages_range: end_index cannot exceed n_configs * n_syms");
+  // kernel arguments are taken at launch, so the range applies to this call only
+  c->row_lo = start_index - 1;
+  c->row_hi = end_index;
+  const int rc = tdscha_perturb_averages_dev(ctx, n_chains, R1_dev, Y1_dev, apply_d4, precision,
+                                             f_dev, d2v_dev, stream);
+  c->row_lo = 0;
+  c->row_hi = c->M;
+  return rc;
 }
 
 int tdscha_perturb_averages_batched(tdscha_ctx* ctx, int n_chains, const double* R1,

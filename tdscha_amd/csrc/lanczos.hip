@@ -346,6 +346,11 @@ __global__ void fill_int(int* p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
+__global__ void fill_zero(double* p, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
 }  // namespace tdscha
 
 using namespace tdscha;
@@ -428,6 +433,13 @@ int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, st.k, n_chains, 0);
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, st.active, n_chains, 1);
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, st.ndone, n_chains, 0);
+  // a chain that stops on |b| < tol leaves its last b slot unwritten: the
+  // coefficient arrays start at zero so that slot reads as a stop
+  {
+    const long na = (long)n_chains * n_steps;
+    hipLaunchKernelGGL(fill_zero, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, st.a, na);
+    hipLaunchKernelGGL(fill_zero, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, st.b, na);
+  }
   hipLaunchKernelGGL(build_mask, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, len,
                      st.mask);
   const long ps = (long)(n_steps + 1) * kRedBlocks;

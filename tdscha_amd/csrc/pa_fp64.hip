@@ -65,13 +65,13 @@ __global__ __launch_bounds__(256) void weights_f64(
     const double* __restrict__ U, const double* __restrict__ F,
     const double* __restrict__ Y1all, const double* __restrict__ R1all,
     const double* __restrict__ fy, const double* __restrict__ fpsi,
-    const double* __restrict__ rho_img, int np, long Mp, int apply_d4,
-    double* __restrict__ coef_all) {
+    const double* __restrict__ rho_img, int np, long Mp, int apply_d4, long tile0, long row_lo,
+    long row_hi, double* __restrict__ coef_all) {
   const int chain = blockIdx.y;
   const double* Y1 = Y1all + (long)chain * np * np;
   const double* R1 = R1all + (long)chain * np;
   double* coef = coef_all + (long)chain * Mp * 4;
-  const long r0 = (long)blockIdx.x * 64;
+  const long r0 = (tile0 + blockIdx.x) * 64;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
@@ -179,7 +179,8 @@ __global__ __launch_bounds__(256) void weights_f64(
     const double s2 = rowsum[tid][1];  // u.Y1.g  (Y1 symmetrised on upload)
     const double s3 = rowsum[tid][2];  // a.R1
     const double s4 = rowsum[tid][3];  // f.R1
-    const double rho = rho_img[row];
+    // images outside [row_lo, row_hi) carry no weight (image-range calls)
+    const double rho = (row >= row_lo && row < row_hi) ? rho_img[row] : 0.0;
     double* c = coef + row * 4;
     c[0] = rho * (-s3 / 3.0 + apply_d4 * s1 / 8.0);
     c[1] = rho * (-s4 / 3.0 + apply_d4 * s2 / 4.0);
@@ -196,8 +197,8 @@ constexpr int WB_KC = 16;
 __global__ __launch_bounds__(256) void outer_f64(
     const double* __restrict__ U, const double* __restrict__ F,
     const double* __restrict__ coef_all, const double* __restrict__ fy, int np, long Mp,
-    int ksplit, int ntile, long rows_per_split, double* __restrict__ zpart_all,
-    double* __restrict__ fpart_all) {
+    int ksplit, int ntile, long kbase, long kend, long rows_per_split, int accumulate,
+    double* __restrict__ zpart_all, double* __restrict__ fpart_all) {
   const int id = blockIdx.x;
   const int xcd = id & 7;
   const int slot = id >> 3;
@@ -228,9 +229,9 @@ __global__ __launch_bounds__(256) void outer_f64(
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
 
-  const long kb = (long)ks * rows_per_split;
+  const long kb = kbase + (long)ks * rows_per_split;
   long ke = kb + rows_per_split;
-  if (ke > Mp) ke = Mp;
+  if (ke > kend) ke = kend;
   const int col = tid & 127;
   const int rsub = tid >> 7;  // 0 or 1
   const double fyn = fy[nu0 + col];
@@ -275,12 +276,18 @@ __global__ __launch_bounds__(256) void outer_f64(
       for (int r = 0; r < 4; ++r) {
         const int mu = mu0 + 64 * wm + 16 * i + 4 * r + lg;
         const int nu = nu0 + 64 * wn + 16 * j + lr;
-        zpart[(long)mu * np + nu] = acc[i][j][r];
+        // accumulate: add to the partial of earlier image chunks (each
+        // element has one writer, chunks run in order)
+        const double prev = accumulate ? zpart[(long)mu * np + nu] : 0.0;
+        zpart[(long)mu * np + nu] = prev + acc[i][j][r];
       }
   if (diag) {
     sF[rsub][col] = fl;
     __syncthreads();
-    if (tid < kTile) fpart[nu0 + tid] = sF[0][tid] + sF[1][tid];
+    if (tid < kTile) {
+      const double prev = accumulate ? fpart[nu0 + tid] : 0.0;
+      fpart[nu0 + tid] = prev + (sF[0][tid] + sF[1][tid]);
+    }
   }
 }
 
@@ -317,15 +324,21 @@ __global__ void reduce_ksplit(const double* __restrict__ zpart_all,
 
 int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* ev) {
   const int ntile = c->np / kTile;
-  dim3 ga((unsigned)(c->Mp / 64), nc);
+  // image range [row_lo, row_hi): whole 64-row tiles of the weights kernel
+  // (rows outside the range get zero coefficients), and the outer GEMM's K
+  // range from the 16-row step below row_lo to the step above row_hi
+  const long t0 = c->row_lo / 64, t1 = (c->row_hi + 63) / 64;
+  dim3 ga((unsigned)(t1 - t0), nc);
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(weights_f64, ga, dim3(256), 0, s, c->U, c->F, c->Y1, c->R1, c->fy, c->fpsi,
-                     c->rho_img, c->np, c->Mp, apply_d4, c->coef);
+                     c->rho_img, c->np, c->Mp, apply_d4, t0, c->row_lo, c->row_hi, c->coef);
   if (ev) (void)hipEventRecord(ev[1], s);
-  const long rps = round_up_l((c->Mp + c->ksplit - 1) / c->ksplit, WB_KC);
+  const long kb = c->row_lo / WB_KC * WB_KC;
+  const long ke = round_up_l(c->row_hi, WB_KC);
+  const long rps = round_up_l((ke - kb + c->ksplit - 1) / c->ksplit, WB_KC);
   dim3 gb((unsigned)(ntile * ntile * c->ksplit), nc);
   hipLaunchKernelGGL(outer_f64, gb, dim3(256), 0, s, c->U, c->F, c->coef, c->fy, c->np, c->Mp,
-                     c->ksplit, ntile, rps, c->zpart, c->fpart);
+                     c->ksplit, ntile, kb, ke, rps, 0, c->zpart, c->fpart);
   if (ev) (void)hipEventRecord(ev[2], s);
   const long nn = (long)c->np * c->np;
   const double norm = 1.0 / (c->n_eff * c->n_syms);

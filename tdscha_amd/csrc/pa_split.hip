@@ -513,8 +513,8 @@ __global__ __launch_bounds__(256, 1) void weights_split(
 // the 16 images of a K step, all chains of a step contiguous).
 __global__ void finalize_coef(const double* __restrict__ part, const double* __restrict__ rho_img,
                               const double* __restrict__ ysc, long Mp, int nct, int apply_d4,
-                              double* __restrict__ coef_all, float* __restrict__ coefB,
-                              double* __restrict__ pmax) {
+                              long row_lo, long row_hi, double* __restrict__ coef_all,
+                              float* __restrict__ coefB, double* __restrict__ pmax) {
   const int chain = blockIdx.y;
   const double isy = 1.0 / ysc[chain];   // the V-weighted sums carry the Y1 scale
   double m1 = 0.0, m2 = 0.0;
@@ -527,7 +527,8 @@ __global__ void finalize_coef(const double* __restrict__ part, const double* __r
     }
     s[0] *= isy;
     s[1] *= isy;
-    const double rho = rho_img[row];
+    // images outside [row_lo, row_hi) carry no weight (image-range calls)
+    const double rho = (row >= row_lo && row < row_hi) ? rho_img[row] : 0.0;
     const double c1 = rho * (-s[2] / 3.0 + apply_d4 * s[0] / 8.0);
     const double c2 = rho * (-s[3] / 3.0 + apply_d4 * s[1] / 4.0);
     const double al = -rho * s[0] / 6.0, be = -rho * s[1] / 3.0;
@@ -1006,7 +1007,8 @@ static int launch_all(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* e
   const long fin_blocks_l = (Mp + 255) / 256;
   const int fin_blocks = fin_blocks_l < kPartBlocks ? (int)fin_blocks_l : kPartBlocks;
   hipLaunchKernelGGL(finalize_coef, dim3((unsigned)fin_blocks, nc), dim3(256), 0, s, c->wpart,
-                     c->rho_img, c->ysc, Mp, nct, apply_d4, (double*)nullptr, c->coef32, c->pmax);
+                     c->rho_img, c->ysc, Mp, nct, apply_d4, c->row_lo, c->row_hi,
+                     (double*)nullptr, c->coef32, c->pmax);
   hipLaunchKernelGGL(bscale_kernel, dim3(nc), dim3(256), 0, s, c->pmax, fin_blocks, st.umax,
                      st.fmax, c->fy, np, c->cmax, c->sbsc);
   hipLaunchKernelGGL(scale_coef_kernel, dim3((unsigned)((Mp + 255) / 256), nc), dim3(256), 0, s,

@@ -129,8 +129,57 @@ class Engine:
             f_dev.ptr, d2v_dev.ptr, None)
         _lib.check(rc, "tdscha_perturb_averages_dev")
 
+    def perturb_averages_range(self, R1, Y1, start_index, end_index, apply_d4=1,
+                               precision=FP64):
+        """Partial averages over images start_index .. end_index (1-based,
+        inclusive, image (config - 1) * n_syms + sym), divided by the global
+        N_eff * n_syms as get_perturb_averages_sym (Modules/tdscha_core.jl:
+        164-238).  R1 (nc, n) or (n,), Y1 (nc, n, n) or (n, n), host arrays."""
+        n = self.n_modes
+        R1 = np.ascontiguousarray(R1, np.float64)
+        Y1 = np.ascontiguousarray(Y1, np.float64)
+        single = R1.ndim == 1
+        R1 = R1.reshape(-1, n)
+        Y1 = Y1.reshape(-1, n, n)
+        nc = R1.shape[0]
+        if Y1.shape[0] != nc:
+            raise ValueError("R1 and Y1 must hold the same number of perturbations")
+        bufs = [_lib.DeviceBuffer.from_array(R1), _lib.DeviceBuffer.from_array(Y1),
+                _lib.DeviceBuffer((nc, n)), _lib.DeviceBuffer((nc, n, n))]
+        try:
+            rc = self._L.tdscha_perturb_averages_range(
+                self.ctx, nc, bufs[0].ptr, bufs[1].ptr, int(apply_d4), precision_code(precision),
+                int(start_index), int(end_index), bufs[2].ptr, bufs[3].ptr, None)
+            _lib.check(rc, "tdscha_perturb_averages_range")
+            self.synchronize()
+            f, d2v = bufs[2].numpy(), bufs[3].numpy()
+        finally:
+            for b in bufs:
+                b.free()
+        return (f[0], d2v[0]) if single else (f, d2v)
+
+    def set_ksplit(self, ksplit):
+        """Number of image ranges of the FP64 outer-product GEMM (multiple of 8)."""
+        _lib.check(self._L.tdscha_set_ksplit(self.ctx, int(ksplit)), "tdscha_set_ksplit")
+        self.ksplit = int(ksplit)
+
 
 _engine_cache = {}
+
+
+def _digest(arrays):
+    """Content hash of a sequence of arrays (shape, dtype and bytes of each)."""
+    try:
+        import xxhash
+        h = xxhash.xxh3_128()
+    except ImportError:
+        import hashlib
+        h = hashlib.blake2b(digest_size=16)
+    for a in arrays:
+        a = np.ascontiguousarray(a)
+        h.update(repr((a.shape, a.dtype.str)).encode())
+        h.update(a.view(np.uint8).reshape(-1))
+    return h.hexdigest()
 
 
 def GetPerturbAverageSym(X, Y, w, rho, R1, Y1, T, apply_d4, n_syms, symmetries, N_degeneracy,
@@ -138,8 +187,11 @@ def GetPerturbAverageSym(X, Y, w, rho, R1, Y1, T, apply_d4, n_syms, symmetries, 
     """Same arguments and in-place outputs as sscha_HP_odd.GetPerturbAverageSym.
 
     (CModules/odd_corr_module.c:491-634.)  The ensemble is uploaded once and
-    cached per (X, Y) buffer; later calls only move R1/Y1 and the outputs.
-    Invalid arguments raise ValueError (the reference calls exit()).
+    cached by the content of everything that defines it (X, Y, w, rho, T, the
+    symmetry blocks, degenerate spaces and block ids); later calls with the
+    same content only move R1/Y1 and the outputs, and any change to those
+    arrays (in place or not) uploads the new ensemble.  Invalid arguments
+    raise ValueError (the reference calls exit()).
     """
     if apply_d4 not in (0, 1):
         raise ValueError("the apply_D4 value passed must be either 0 or 1 (got %r)" % (apply_d4,))
@@ -147,8 +199,12 @@ def GetPerturbAverageSym(X, Y, w, rho, R1, Y1, T, apply_d4, n_syms, symmetries, 
         raise ValueError("N_modes from X is %d, while len(w) = %d" % (np.shape(X)[1], len(w)))
     if int(n_syms) != np.shape(symmetries[0])[0]:
         raise ValueError("n_syms does not match the symmetry blocks")
-    key = (X.ctypes.data, Y.ctypes.data, np.shape(X), float(T), int(n_syms),
-           rho.ctypes.data, w.ctypes.data)
+    key = (float(T), int(n_syms),
+           _digest([np.asarray(X, np.float64), np.asarray(Y, np.float64),
+                    np.asarray(w, np.float64), np.asarray(rho, np.float64),
+                    np.asarray(sym_block_id, np.int32)]
+                   + [np.asarray(s, np.float64) for s in symmetries]
+                   + [np.asarray(b, np.int32) for b in degenerate_space]))
     eng = _engine_cache.get(key)
     if eng is None:
         eng = Engine(X, Y, w, rho, T, symmetries, degenerate_space, sym_block_id)

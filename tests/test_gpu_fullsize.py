@@ -1,23 +1,32 @@
-"""The L application at the full workload size (128-atom rock salt, 381
-modes, 20000 configurations x 48 point-group images = 960000 images, the
-translational projector), where the oracle is too slow: properties that do
-not depend on size.
+"""The perturbed averages and the L application at the full workload size
+(128-atom rock salt, 381 modes, 20000 configurations x 48 point-group images
+= 960000 images, the translational projector).
 
+- FP64 perturbed averages against the oracle's OpenMP C restatement on all
+  host cores, term by term: (R1, 0) gives the D3 d2v term alone (f is zero),
+  (0, Y1) gives the D3 force term and the D4 d2v term (norm-wise relative
+  1e-12, the bar of the FP64 path);
+- symmetry of the whole L (harmonic, D3, D4 and the projector) in the Wigner
+  metric on Gamma-sector Krylov vectors: for psi0 the IR perturbation and
+  v_k = L^k psi0, <v_i, L v_j>_M = <L v_i, v_j>_M (relative 1e-11).  The
+  symmetric Lanczos recurrence (DL:5958-6491, run_simm) relies on it.  On
+  vectors outside the Gamma sector the projector form is not symmetric (the
+  projector acts on the averages only), so only the sector the chains live
+  in is tested;
 - linearity: L(s x + t y) = s L x + t L y in FP64 (relative 1e-12 of |L|);
-- symmetry in the Wigner metric of the harmonic + D4 part: <x, L y> =
-  <L x, y> (relative 1e-11).  The symmetric Lanczos recurrence (DL:5958-6491,
-  run_simm) assumes it.  With the translational projector of this workload
-  the reference's D3 term is not symmetric (its projector acts on the
-  average force only; the oracle gives 1.5e-4 at N = 200, 48 images, and
-  exact symmetry with all 384 operations), so that term is covered by
-  linearity and the split-vs-FP64 check;
 - the split FP16/BF16 hi/lo products against FP64: relative L2 error of L x
   at most 1e-5 (the spectrum tolerance of the split path is 1e-4).
 """
+import os
+
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
 @pytest.fixture(scope="module")
@@ -34,34 +43,71 @@ def full():
     n = len(lanc.psi)
     X = rng.standard_normal((3, n))
     X[2] = 0.4 * X[0] - 1.7 * X[1]
-    yield lanc, X
+    yield lanc, X, ens
     lanc.close()
 
 
-def test_full_size_linearity_and_d4_wigner_symmetry(full):
-    lanc, X = full
+@pytest.mark.parametrize("sector", ["R1", "Y1"])
+def test_full_size_fp64_perturb_averages_vs_oracle(full, sector):
+    from oracle import c_oracle as CO
+    lanc, _, ens = full
+    n = lanc.n_modes
+    rng = np.random.default_rng(21)
+    R1 = np.zeros(n)
+    Y1 = np.zeros((n, n))
+    if sector == "R1":
+        R1 = rng.standard_normal(n)
+    else:
+        Y1 = rng.standard_normal((n, n)) * 1e-3
+        Y1 = Y1 + Y1.T
+    f, d = lanc.engine.perturb_averages(R1, Y1, 1)
+    threads = min(16, os.cpu_count() or 1)
+    print("oracle: %d threads over 960000 images (%s sector)" % (threads, sector), flush=True)
+    fo, do = CO.oracle_perturb_averages(lanc.X, lanc.Y, lanc.w, lanc.rho, R1, Y1, lanc.T, 1,
+                                        lanc.symmetries, lanc.degenerate_space,
+                                        lanc.sym_block_id, n_threads=threads)
+    ed = rel(d, do)
+    print("%s: rel d2v %.2e" % (sector, ed))
+    assert ed < 1e-12
+    if sector == "R1":
+        assert np.max(np.abs(f)) == 0.0 and np.max(np.abs(fo)) == 0.0
+    else:
+        ef = rel(f, fo)
+        print("%s: rel f %.2e" % (sector, ef))
+        assert ef < 1e-12
+
+
+def test_full_size_gamma_krylov_symmetry_and_linearity(full):
+    lanc, X, ens = full
     lanc.precision = "fp64"
     LX = lanc.apply_L_batch(X)
     scale = np.linalg.norm(LX[:2], axis=1).max()
     lin = np.linalg.norm(LX[2] - (0.4 * LX[0] - 1.7 * LX[1])) / scale
-    lanc.ignore_v3 = True
-    try:
-        L4 = lanc.apply_L_batch(X[:2])
-    finally:
-        lanc.ignore_v3 = False
-    m = lanc.mask_dot_wigner()
-    xy = np.dot(X[0] * m, L4[1])
-    yx = np.dot(L4[0] * m, X[1])
-    ref = np.sqrt(np.dot(X[0] * m, X[0]) * np.dot(L4[1] * m, L4[1]))
-    sym = abs(xy - yx) / ref
-    print("linearity %.2e  symmetry %.2e (<x,Ly> = %.6e)" % (lin, sym, xy))
+    print("linearity %.2e" % lin)
     assert lin < 1e-12
-    assert sym < 1e-11
+    lanc.prepare_perturbation(ens.ir_vector((1.0, 0.0, 0.0)), masses_exp=-1)
+    v = [lanc.psi.copy()]
+    for _ in range(3):
+        v.append(lanc.apply_L_batch(v[-1][None, :])[0])
+    m = lanc.mask_dot_wigner()
+
+    def dot(a, b):
+        return np.dot(a * m, b)
+
+    worst = 0.0
+    for i, j in [(0, 1), (0, 2), (1, 2)]:
+        # <v_i, L v_j> = <v_i, v_(j+1)> against <L v_i, v_j> = <v_(i+1), v_j>
+        lhs, rhs = dot(v[i], v[j + 1]), dot(v[i + 1], v[j])
+        ref = np.sqrt(dot(v[i], v[i]) * dot(v[j + 1], v[j + 1]))
+        worst = max(worst, abs(lhs - rhs) / ref)
+        print("<v%d, L v%d> = %.12e  <L v%d, v%d> = %.12e" % (i, j, lhs, i, j, rhs))
+    print("Gamma-sector Krylov symmetry %.2e" % worst)
+    assert worst < 1e-11
 
 
 @pytest.mark.parametrize("prec", ["fp16x2", "bf16x2"])
 def test_full_size_split_against_fp64(full, prec):
-    lanc, X = full
+    lanc, X, _ = full
     lanc.precision = "fp64"
     ref = lanc.apply_L_batch(X[:2])
     lanc.precision = prec
