@@ -104,6 +104,36 @@ int tdscha_perturb_averages_range(tdscha_ctx* ctx, int n_chains, const double* R
                                   long start_index, long end_index, double* f_dev,
                                   double* d2v_dev, void* stream);
 
+/*
+ * Variant of tdscha_perturb_averages_range (FP64, one perturbation) that
+ * allocates nothing: no context, no hipMalloc/hipFree, no allocating library
+ * call.  It has no counterpart in the reference; it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  Every argument is a device pointer,
+ * with the arrays of get_perturb_averages_sym (Modules/tdscha_core.jl:212-238)
+ * in the packed layout of Modules/DynamicalLanczos.py:807-822:
+ *   X, Y [n_configs, n_modes] row-major; w [n_modes]; rho [n_configs];
+ *   R1 [n_modes]; Y1 [n_modes, n_modes];
+ *   syms [n_blocks, n_syms, dmax, dmax] (row = output mode);
+ *   n_degeneracy [n_blocks]; degenerate_space [n_blocks, dmax];
+ *   sym_block_id [n_modes];  f_out [n_modes], d2v_out [n_modes, n_modes].
+ * start_index, end_index as tdscha_perturb_averages_range (1-based,
+ * inclusive); normalised by the global sum(rho) * n_syms.  The work is
+ * queued on `stream` (NULL = the null stream) and the call returns at once.
+ * Intermediates live in one 64 MiB device array per GPU, held until the
+ * process ends: n_modes up to 768 (larger gives 2).  Calls that overlap on
+ * one GPU (other threads or streams) share that array and race; serialise
+ * them.  The tables are trusted as the reference trusts them: entries
+ * outside the tables are skipped, never read.
+ */
+int tdscha_perturb_averages_noalloc(int n_modes, int n_configs, const double* X, const double* Y,
+                                    const double* w, const double* rho, const double* R1,
+                                    const double* Y1, double T, int apply_d4, int n_syms,
+                                    int n_blocks, int dmax, const double* syms,
+                                    const int* n_degeneracy, const int* degenerate_space,
+                                    const int* sym_block_id, long start_index, long end_index,
+                                    double* f_out, double* d2v_out, void* stream);
+
 /* Batched host-pointer variant (same layouts as the _dev entry, synchronous). */
 int tdscha_perturb_averages_batched(tdscha_ctx* ctx, int n_chains, const double* R1,
                                     const double* Y1, int apply_d4, int precision, double* f_out,

@@ -37,6 +37,10 @@ def _signatures(L):
     L.tdscha_perturb_averages_batched.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, _vp, _vp]
     L.tdscha_perturb_averages_range.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int,
                                                 C.c_long, C.c_long, _vp, _vp, _vp]
+    L.tdscha_perturb_averages_noalloc.argtypes = [C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                  C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                  _vp, _vp, _vp, _vp, C.c_long, C.c_long, _vp,
+                                                  _vp, _vp]
     L.tdscha_set_translations.argtypes = [_vp, C.c_int, _vp, C.c_int, _vp, _vp]
     L.tdscha_apply_L.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                  C.c_int, C.c_int, C.c_double, C.c_int]
