@@ -1,0 +1,120 @@
+"""Pins of the oracle's symmetric Lanczos with full reorthogonalisation
+(oracle/lanczos_oracle.py:run_symmetric_full_reorth), the checker of the
+device recurrence's 50-coefficient gate (CPU).
+
+Each check is chosen so that a plausible slip fails it:
+- before orthogonality is lost, the coefficients equal the reference's own
+  recurrence (run_FT with run_simm, DL:6145-6240, restated and pinned to the
+  golden run elsewhere) and the q-space oracle's Hermitian Lanczos with
+  modified Gram-Schmidt (QL:967-1182) on a real vector;
+- the basis is orthonormal in the mask metric after many steps (a dropped
+  Gram-Schmidt pass or metric fails);
+- the three-term relation L q_k = b_(k-1) q_(k-1) + a_k q_k + b_k q_(k+1)
+  holds for every k (a shifted index or sign of a or b fails);
+- on a dense explicit-tensor L (DL:5167-5900 restated) small enough for
+  exact linear algebra, the resolvent of the tridiagonal matrix equals the
+  dense resolvent <q0, (z - L)^(-1) q0>_M and, when the Krylov space is the
+  whole space, the Ritz values equal the eigenvalues of L.
+"""
+import numpy as np
+
+from oracle import lanczos_oracle as O
+from oracle import qspace_oracle as QO
+from tests.systems import random_block_system
+
+
+def _rocksalt16(N=200, seed=0):
+    from tdscha_amd.rocksalt import Ensemble, RockSalt
+    ens = Ensemble(RockSalt(L=2), 300.0, N, seed=seed)
+    ms = ens.symmetries(use_symmetries=True, gamma_only=False)
+    sysO = O.OracleSystem(ens.X, ens.Y, ens.w, ens.rho, ens.T, ms.dense(), use_wigner=True)
+    n = len(ens.w)
+    psi = np.zeros(O.len_psi(n))
+    psi[:n] = np.einsum("a,a,ab->b", 1 / np.sqrt(ens.m), ens.ir_vector((1.0, 0.0, 0.0)),
+                        ens.pols)
+    return sysO, psi, O.mask_dot_wigner(n)
+
+
+def test_early_steps_equal_the_reference_recurrence_and_mgs_lanczos():
+    """A random start vector without symmetry images spans a large Krylov
+    space: over 40 steps the plain recurrence stays orthogonal to 1e-11, so
+    all three recurrences must give the same coefficients."""
+    from tdscha_amd.rocksalt import Ensemble, RockSalt
+    ens = Ensemble(RockSalt(L=2), 300.0, 200, seed=0)
+    n = len(ens.w)
+    sysO = O.OracleSystem(ens.X, ens.Y, ens.w, ens.rho, ens.T, None, use_wigner=True)
+    m = O.mask_dot_wigner(n)
+    psi = np.random.default_rng(5).normal(size=O.len_psi(n))
+    n_it = 40
+    a, b, Q = O.run_symmetric_full_reorth(sysO.apply_full_L, psi, n_it, m)
+    st = O.run_FT(sysO.apply_full_L, None, psi, n_it, run_simm=True, mask=m)
+    Qn = np.array(st.Q)
+    assert np.max(np.abs((Qn * m) @ Qn.T - np.eye(len(Qn)))) < 1e-11
+    ra = np.max(np.abs(a - np.array(st.a)) / np.abs(np.array(st.a)))
+    sb = np.array(st.b)
+    assert len(sb) == n_it and len(b) == n_it
+    rb = np.max(np.abs(b - sb) / np.abs(sb))
+    assert ra < 1e-12 and rb < 1e-12, (ra, rb)
+    qa, qb, qc, _ = QO.hermitian_lanczos(lambda v: sysO.apply_full_L(v.real).astype(complex),
+                                         psi.astype(complex), m, n_it, mgs=True)
+    ra = np.max(np.abs(a - qa) / np.abs(qa))
+    rb = np.max(np.abs(b[:n_it - 1] - qb[:n_it - 1]) / np.abs(qb[:n_it - 1]))
+    assert ra < 1e-12 and rb < 1e-12, (ra, rb)
+    assert np.allclose(qb, qc, rtol=1e-12)
+
+
+def test_orthonormal_basis_and_three_term_relation_over_many_steps():
+    """The IR chain under the full group spans a small invariant space (b
+    drops to 3e-10 at step 9): past it the plain recurrence loses
+    orthogonality, the reorthogonalised one keeps it."""
+    sysO, psi, m = _rocksalt16()
+    n_it = 30
+    a, b, Q = O.run_symmetric_full_reorth(sysO.apply_full_L, psi, n_it, m)
+    assert len(a) == n_it and len(b) == n_it and Q.shape[0] == n_it + 1
+    G = (Q * m) @ Q.T
+    assert np.max(np.abs(G - np.eye(len(Q)))) < 1e-10
+    nrm = max(abs(a).max(), abs(b).max())
+    for k in range(n_it):
+        Lq = sysO.apply_full_L(Q[k])
+        r = Lq - a[k] * Q[k] - b[k] * Q[k + 1]
+        if k > 0:
+            r -= b[k - 1] * Q[k - 1]
+        assert np.sqrt(np.dot(r * m, r)) < 1e-9 * nrm, k
+    # without reorthogonalisation the same chain loses orthogonality: the
+    # reorthogonalised recurrence is what this length needs
+    st = O.run_FT(sysO.apply_full_L, None, psi, n_it, run_simm=True, mask=m)
+    Qn = np.array(st.Q)
+    Gn = (Qn * m) @ Qn.T
+    assert np.max(np.abs(Gn - np.eye(len(Qn)))) > 1e-6
+
+
+def test_tridiagonal_resolvent_and_ritz_values_on_a_dense_L():
+    s = random_block_system(3, 30, 1, [1] * 3, seed=5, T=200.0)
+    s["Y"] *= 1e5
+    sysO = O.OracleSystem(s["X"] * 1e-1, s["Y"], s["w"] * 1e2, s["rho"], s["T"] * 10, None,
+                          use_wigner=True)
+    L = O.dense_L(sysO)
+    n = 3
+    m = O.mask_dot_wigner(n)
+    dim = O.len_psi(n)
+    # symmetric in the metric: M L = (M L)^T
+    ML = m[:, None] * L
+    assert np.max(np.abs(ML - ML.T)) < 1e-12 * np.max(np.abs(ML))
+    psi = np.random.default_rng(3).normal(size=dim)
+    a, b, Q = O.run_symmetric_full_reorth(lambda v: L @ v, psi, dim, m, tol=1e-10)
+    k = len(a)
+    T = np.diag(a) + np.diag(b[:k - 1], 1) + np.diag(b[:k - 1], -1)
+    q0 = psi / np.sqrt(np.dot(psi * m, psi))
+    for z in (0.3 + 0.2j, -1.7 + 0.05j, 2.5 - 0.4j):
+        z = z * np.max(np.abs(np.linalg.eigvals(L)))
+        g_dense = np.dot(q0 * m, np.linalg.solve(z * np.eye(dim) - L, q0))
+        g_tri = np.linalg.inv(z * np.eye(k) - T)[0, 0]
+        assert abs(g_tri - g_dense) < 1e-10 * abs(g_dense), (z, g_tri, g_dense)
+    ev_L = np.sort(np.linalg.eigvals(L).real)
+    ev_T = np.sort(np.linalg.eigvalsh(T))
+    scale = np.max(np.abs(ev_L))
+    if k == dim:
+        assert np.max(np.abs(ev_T - ev_L)) < 1e-10 * scale
+    else:   # a smaller invariant space: every Ritz value is an eigenvalue of L
+        for e in ev_T:
+            assert np.min(np.abs(ev_L - e)) < 1e-10 * scale
