@@ -86,6 +86,11 @@ class QSystem:
         self.ignore_v3 = ignore_v3
         self.ignore_v4 = ignore_v4
         self.n_syms = 1
+        # a deliberate slip planted by the tests that check the pins fail on
+        # it: None (the restatement), "lw" (conj on x_q2 instead of x_q1),
+        # "bu" (conj(x_q2) in buffer_u), "d2v" (conj on the first factor of
+        # the d2v products)
+        self._slip = None
         nb = self.nb
         self.block_sizes = [nb * nb if a < b else nb * (nb + 1) // 2
                             for a, b in self.unique_pairs]
@@ -201,12 +206,16 @@ class QSystem:
         wRf = rho / 3.0 * (np.conj(R1)[None] * yp).sum(1)
         bu = np.zeros_like(x)
         tot = np.zeros(self.N, np.complex128)
+        slip = self._slip
         for p, (iq1, iq2) in enumerate(self.unique_pairs):
             A = alpha1[p]
-            bu[iq1] += x[iq2] @ A.T
+            bu[iq1] += (np.conj(x[iq2]) if slip == "bu" else x[iq2]) @ A.T
             if iq1 != iq2:
                 bu[iq2] += x[iq1] @ A.conj()
-            lw = np.einsum("ia,ab,ib->i", np.conj(x[iq1]), A, x[iq2])
+            if slip == "lw":
+                lw = np.einsum("ia,ab,ib->i", x[iq1], A, np.conj(x[iq2]))
+            else:
+                lw = np.einsum("ia,ab,ib->i", np.conj(x[iq1]), A, x[iq2])
             tot += lw + np.conj(lw) if iq1 < iq2 else lw
         bfw = np.einsum("qia,aq,qia->i", np.conj(bu), fP, y)
         w1 = -tot / 2.0 * rho / 3.0
@@ -225,6 +234,8 @@ class QSystem:
             r21 = y[iq1]
             r12 = fY[:, iq2][None] * x[iq2]
             r22 = y[iq2]
+            if slip == "d2v":
+                r11, r21, r12, r22 = np.conj(r11), np.conj(r21), np.conj(r12), np.conj(r22)
             D = -np.einsum("i,ia,ib->ab", wc, r11, np.conj(r22))
             D -= np.einsum("i,ia,ib->ab", wc, r21, np.conj(r12))
             D -= np.einsum("i,ia,ib->ab", wd, r11, np.conj(r12))
