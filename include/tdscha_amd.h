@@ -167,6 +167,17 @@ int tdscha_apply_L_dev(tdscha_ctx* ctx, int n_vec, const double* psi_dev, double
                        int ignore_v4, int reverse, double shift, int precision, void* stream);
 
 /*
+ * Inverse of the harmonic part in the standard representation, the
+ * preconditioner behind Lanczos.M_linop (replaces
+ * Lanczos.apply_L1_inverse_FT, Modules/DynamicalLanczos.py:2451-2549, with
+ * its regularisation of singular and unpopulated pair blocks): n_vec (1..8)
+ * host vectors of length n + n(n+1); transpose applies the transposed
+ * blocks; ignore_harmonic gives zero.
+ */
+int tdscha_apply_L1_inverse(tdscha_ctx* ctx, int n_vec, const double* psi, double* out,
+                            int transpose, int ignore_harmonic);
+
+/*
  * Device-resident symmetric Lanczos in the Wigner representation
  * (run_FT with run_simm, Modules/DynamicalLanczos.py:6032-6240) for n_chains
  * (1..8) starting vectors psi0 [n_chains, n + n(n+1)] (host), batched through

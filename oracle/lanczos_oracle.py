@@ -208,6 +208,53 @@ def harmonic_L1(psi, w, T, use_wigner, transpose=False, ignore_harmonic=False):
     return out
 
 
+def harmonic_L1_inverse(psi, w, T, transpose=False, ignore_harmonic=False):
+    """apply_L1_inverse_FT (DL:2451-2549), the preconditioner of M_linop:
+    the inverse of the standard-representation harmonic blocks with the
+    reference's regularisation (populations zeroed where 0.01 w_a RY_TO_K > T
+    for both members, DL:2491-2498; determinant below 1e-12 keeps -1/X only,
+    DL:2522-2531; equal thermally populated frequencies invert Y1 alone,
+    DL:2533-2536)."""
+    n = len(w)
+    out = np.zeros_like(psi)
+    if ignore_harmonic:
+        return out
+    r, c = upper_pairs(n)
+    w_a, w_b = w[c], w[r]
+    n_a = np.zeros_like(w_a)
+    n_b = np.zeros_like(w_a)
+    if T > 0:
+        keep = ~(0.01 * w_a * RY_TO_K > T)
+        n_a = np.where(keep, 1.0 / (np.exp(w_a / (T / RY_TO_K)) - 1.0), 0.0)
+        n_b = np.where(keep, 1.0 / (np.exp(w_b / (T / RY_TO_K)) - 1.0), 0.0)
+    out[:n] = (psi[:n] / w) / w
+    d = (2 * n_a + 1) * (2 * n_b + 1)
+    X = -w_a ** 2 - w_b ** 2 - (2 * w_a * w_b) / d
+    Y = -(8 * w_a * w_b) / d
+    X1 = -(2 * n_a * n_b + n_a + n_b) * (2 * n_a * n_b + n_a + n_b + 1) * (2 * w_a * w_b) / d
+    Y1 = -w_a ** 2 - w_b ** 2 + (2 * w_a * w_b) / d
+    den = X * Y1 - X1 * Y
+    reg = den < EPSILON
+    safe = np.where(reg, 1.0, den)
+    Xn = np.where(reg, -1.0 / X, -Y1 / safe)
+    Yn = np.where(reg, 0.0, Y / safe)
+    X1n = np.where(reg, 0.0, X1 / safe)
+    Y1n = np.where(reg, 0.0, -X / safe)
+    if T > EPSILON:
+        Y1n = np.where((w_a == w_b) & (n_a > EPSILON), -1.0 / Y1, Y1n)
+    nw2 = len(w_a)
+    sY, sA = n, n + nw2
+    out[sY:sA] = Xn * psi[sY:sA]
+    if not transpose:
+        out[sY:sA] += Yn * psi[sA:]
+        out[sA:] += X1n * psi[sY:sA]
+    else:
+        out[sA:] += Yn * psi[sY:sA]
+        out[sY:sA] += X1n * psi[sA:]
+    out[sA:] += Y1n * psi[sA:]
+    return out
+
+
 # --------------------------------------------------------------------------
 # Wigner change of variables
 # --------------------------------------------------------------------------

@@ -46,6 +46,7 @@ def _signatures(L):
                                  C.c_int, C.c_int, C.c_double, C.c_int]
     L.tdscha_apply_L_dev.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_int, C.c_int, C.c_double, C.c_int, _vp]
+    L.tdscha_apply_L1_inverse.argtypes = [_vp, C.c_int, _vp, _vp, C.c_int, C.c_int]
     L.tdscha_lanczos_run.argtypes = [_vp, C.c_int, _vp, C.c_int, _vp, C.c_double, C.c_int,
                                      C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                      _vp, _vp]

@@ -443,11 +443,116 @@ int apply_L_dev(Ctx* c, int nc, const double* psi, double* out, int use_wigner, 
   return 0;
 }
 
+// Inverse of the harmonic 2x2 pair blocks (Modules/DynamicalLanczos.py:2451-2549),
+// evaluated once per context on the host in FP64 with the reference's
+// regularisation: populations set to zero where 0.01 w_a RY_TO_K > T (the
+// mask of both members uses w_a, DL:2491-2492), a determinant below 1e-12
+// keeps only the inverse of the X element, and at T > 0 equal frequencies
+// with a thermal population invert the Y1 element alone.
+static int setup_inverse(Ctx* c) {
+  if (c->inv_coef) return 0;
+  const int n = c->n;
+  const long P = (long)n * (n + 1) / 2;
+  std::vector<double> wv(n), ic(4 * P);
+  TDS_CHECK_HIP(hipMemcpy(wv.data(), c->w, sizeof(double) * n, hipMemcpyDeviceToHost));
+  const double eps = 1e-12;
+  long p = 0;
+  for (int r = 0; r < n; ++r)
+    for (int col = r; col < n; ++col, ++p) {
+      const double wa = wv[col], wb = wv[r];
+      double na = 0.0, nb = 0.0;
+      if (c->T > 0) {
+        na = 1.0 / (std::exp(wa / (c->T / RY_TO_K)) - 1.0);
+        nb = 1.0 / (std::exp(wb / (c->T / RY_TO_K)) - 1.0);
+        if (0.01 * wa * RY_TO_K > c->T) na = nb = 0.0;
+      }
+      const double d = (2 * na + 1) * (2 * nb + 1);
+      const double X = -wa * wa - wb * wb - (2 * wa * wb) / d;
+      const double Y = -(8 * wa * wb) / d;
+      const double X1 = -(2 * na * nb + na + nb) * (2 * na * nb + na + nb + 1) * (2 * wa * wb) / d;
+      const double Y1 = -wa * wa - wb * wb + (2 * wa * wb) / d;
+      const double den = X * Y1 - X1 * Y;
+      double xn, yn, x1n, y1n;
+      if (den < eps) {       // den -> inf in the reference
+        xn = -1.0 / X;
+        yn = 0.0;
+        x1n = 0.0;
+        y1n = 0.0;
+      } else {
+        xn = -Y1 / den;
+        yn = Y / den;
+        x1n = X1 / den;
+        y1n = -X / den;
+      }
+      if (c->T > eps && wa == wb && na > eps) y1n = -1.0 / Y1;
+      ic[0 * P + p] = xn;
+      ic[1 * P + p] = yn;
+      ic[2 * P + p] = x1n;
+      ic[3 * P + p] = y1n;
+    }
+  TDS_CHECK_HIP(hipMalloc((void**)&c->inv_coef, sizeof(double) * 4 * P));
+  TDS_CHECK_HIP(hipMemcpy(c->inv_coef, ic.data(), sizeof(double) * 4 * P, hipMemcpyHostToDevice));
+  return 0;
+}
+
+// out = L1^-1 psi (or its transpose); one thread per psi entry
+__global__ void apply_l1_inverse(const double* __restrict__ psi_all, long len, int n,
+                                 const double* __restrict__ w, const double* __restrict__ ic,
+                                 int transpose, int ignore_harmonic, double* __restrict__ out_all) {
+  const int chain = blockIdx.y;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= len) return;
+  const double* psi = psi_all + chain * len;
+  double* out = out_all + chain * len;
+  if (ignore_harmonic) {
+    out[e] = 0.0;
+    return;
+  }
+  if (e < n) {
+    out[e] = (psi[e] / w[e]) / w[e];
+    return;
+  }
+  const long P = (long)n * (n + 1) / 2;
+  const bool secY = e < n + P;
+  const long p = secY ? e - n : e - n - P;
+  const double py = psi[n + p], pa = psi[n + P + p];
+  const double xn = ic[p], yn = ic[P + p], x1n = ic[2 * P + p], y1n = ic[3 * P + p];
+  double v;
+  if (secY)
+    v = xn * py + (transpose ? x1n * pa : yn * pa);
+  else
+    v = (transpose ? yn * py : x1n * py) + y1n * pa;
+  out[e] = v;
+}
+
 }  // namespace tdscha
 
 using namespace tdscha;
 
 extern "C" {
+
+int tdscha_apply_L1_inverse(tdscha_ctx* ctx, int n_vec, const double* psi, double* out,
+                            int transpose, int ignore_harmonic) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_apply_L1_inverse: NULL context");
+  TDS_REQUIRE(psi && out, 2, "tdscha_apply_L1_inverse: NULL pointer");
+  TDS_REQUIRE(n_vec >= 1 && n_vec <= kMaxChains, 2, "n_vec must be in [1, 8]");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = setup_pairs(c);
+  if (rc) return rc;
+  rc = setup_inverse(c);
+  if (rc) return rc;
+  const long len = c->n + (long)c->n * (c->n + 1);
+  TDS_CHECK_HIP(hipMemcpyAsync(c->psi_dev, psi, sizeof(double) * n_vec * len,
+                               hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(apply_l1_inverse, dim3((unsigned)((len + 255) / 256), n_vec), dim3(256), 0,
+                     c->stream, c->psi_dev, len, c->n, c->w, c->inv_coef, transpose,
+                     ignore_harmonic, c->psi_out_dev);
+  TDS_CHECK_HIP(hipGetLastError());
+  TDS_CHECK_HIP(hipMemcpyAsync(out, c->psi_out_dev, sizeof(double) * n_vec * len,
+                               hipMemcpyDeviceToHost, c->stream));
+  TDS_CHECK_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
 
 int tdscha_set_translations(tdscha_ctx* ctx, int n_cart, const double* pols, int n_cells,
                             const int* inv_perms, const double* trans_projector) {

@@ -148,6 +148,7 @@ struct Ctx {
   int* pair_col = nullptr;
   // standard representation helpers
   double* std_coef = nullptr;  // [8, npairs] harmonic 2x2 block coefficients
+  double* inv_coef = nullptr;  // [4, npairs] inverse harmonic 2x2 blocks (preconditioner)
   double* yw = nullptr;        // [np] Y_w
   double* reaw = nullptr;      // [np] ReA_w
   // gamma-only translational projector

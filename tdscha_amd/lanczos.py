@@ -117,10 +117,13 @@ class Lanczos(object):
         self.ensemble = None
         self.engine = None
         self._sym_source = None
+        self.L_linop = None
+        self.M_linop = None
         self.__total_attributes__ = [item for item in self.__dict__.keys()]
         self.fixed_attributes = True
         if ensemble is None:
             return
+        self.ignore_small_w = bool(getattr(ensemble, "ignore_small_w", False))
         # mode-basis quantities (DL:322-452); the ensemble object provides them
         self.ensemble = ensemble
         self.T = float(ensemble.T)
@@ -147,6 +150,19 @@ class Lanczos(object):
         self.X = np.ascontiguousarray(X)
         self.Y = np.ascontiguousarray(Y)
         self.psi = np.zeros(self.n_modes + self.n_modes * (self.n_modes + 1), dtype=TYPE_DP)
+        self._make_linops()
+
+    def _make_linops(self):
+        """L and the harmonic preconditioner as SciPy LinearOperators
+        (DL:478-489, rebuilt by load_status as DL:3829-3838)."""
+        import scipy.sparse.linalg
+        size = len(self.psi)
+        self.L_linop = scipy.sparse.linalg.LinearOperator(
+            shape=(size, size), matvec=lambda psi: self.apply_full_L(np.ravel(psi)),
+            rmatvec=lambda psi: self.apply_full_L(np.ravel(psi), transpose=True), dtype=TYPE_DP)
+        self.M_linop = scipy.sparse.linalg.LinearOperator(
+            shape=(size, size), matvec=self.apply_L1_inverse_FT,
+            rmatvec=lambda psi: self.apply_L1_inverse_FT(psi, transpose=True), dtype=TYPE_DP)
 
     def __setattr__(self, name, value):
         """Reject unknown attributes with close-match suggestions (DL:508-526)."""
@@ -250,7 +266,14 @@ class Lanczos(object):
         ec = effective_charges
         if ec is None:
             raise ValueError("Error, no effective charge found. Cannot initialize IR response")
+        ec = np.asarray(ec, dtype=TYPE_DP)
         n_cells = self.nat // ec.shape[0]
+        msg = ("Error, effective charges of the wrong shape: {}\nNumber of modes : {}\n"
+               "Dimension of the supercell : {}".format(ec.shape, self.n_modes, n_cells))
+        if ec.ndim != 3 or ec.shape[1] != 3 or ec.shape[2] != 3:
+            raise ValueError(msg)
+        if not self.ignore_small_w and ec.shape[0] * ec.shape[2] * n_cells != self.n_modes + 3:
+            raise ValueError(msg)     # DL:1739-1741: only the 3 translations are dropped
         z_eff = np.einsum("abc, b", ec, pol_vec)
         self.prepare_perturbation(np.tile(z_eff.ravel(), n_cells), masses_exp=-1)
 
@@ -282,6 +305,23 @@ class Lanczos(object):
         out = self.apply_L_batch(self.psi[None, :], transpose=transpose)[0]
         self.psi = out
         return self.psi
+
+    def apply_L1_inverse_FT(self, psi, transpose=False):
+        """DL:2451-2549: the inverse of the harmonic 2x2 pair blocks of the
+        standard representation (the preconditioner of M_linop), with the
+        reference's regularisation, on the device.  Does not touch self.psi."""
+        if not self.initialized or self.engine is None:
+            raise ValueError("Error, this class must be initialized before lunching the "
+                             "computation:\n Use the .init()")
+        psi = np.ascontiguousarray(np.ravel(psi), dtype=TYPE_DP)
+        if len(psi) != self.n_modes + self.n_modes * (self.n_modes + 1):
+            raise ValueError("ERROR,\nThe initial vector for the Lanczos algorithm has a wrong "
+                             "dimension.")
+        out = np.empty_like(psi)
+        rc = _lib.lib().tdscha_apply_L1_inverse(self.engine.ctx, 1, _p(psi), _p(out),
+                                                int(bool(transpose)), int(self.ignore_harmonic))
+        _lib.check(rc, "tdscha_apply_L1_inverse")
+        return out
 
     def L_matvec(self, v):
         return self.apply_full_L(np.array(v, dtype=TYPE_DP))
@@ -536,6 +576,7 @@ class Lanczos(object):
             self.perturbation_modulus = float(data["perturbation_modulus"])
         self.n_syms = int(data.get("n_syms", 1))
         self.initialized = False
+        self._make_linops()
 
     def prepare_input_files(self, root_name="tdscha", n_steps=100, start_from_scratch=True,
                             directory=".", run_symm=False):

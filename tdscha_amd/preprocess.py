@@ -60,8 +60,10 @@ def symmetrize_unit_cell_vector(v, rotations, atom_maps):
 class PreparedEnsemble:
     """Mode-space ensemble (the attributes the Lanczos constructor reads)."""
 
-    def __init__(self, T, w, pols, m, rho, X, Y, u_tilde, f_tilde, mode_mask, structure=None):
+    def __init__(self, T, w, pols, m, rho, X, Y, u_tilde, f_tilde, mode_mask, structure=None,
+                 ignore_small_w=False):
         self.T = float(T)
+        self.ignore_small_w = bool(ignore_small_w)
         self.w = w
         self.pols = pols
         self.m = m
@@ -161,7 +163,8 @@ def prepare_ensemble(u_disps, forces, sscha_forces, masses, w, pols, T, rho=None
             lat = lat / BOHR_ANGSTROM
             pos = pos / BOHR_ANGSTROM
         structure = (lat, pos, np.asarray(spec))
-    return PreparedEnsemble(T, w[good].copy(), pg, m, rho, X, Y, u, f, good, structure)
+    return PreparedEnsemble(T, w[good].copy(), pg, m, rho, X, Y, u, f, good, structure,
+                            ignore_small_w=ignore_small_w)
 
 
 def prepare_from_files(dyn_prefix, nqirr, ensemble_dir, population, T, rho=None,

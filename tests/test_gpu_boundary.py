@@ -164,6 +164,35 @@ def test_drop_in_cache_sees_changed_inputs():
     check(s2["syms"])         # other symmetry matrices, same n_syms and block layout
 
 
+def test_preconditioner_and_linear_operators(snte):
+    """apply_L1_inverse_FT on the device against the oracle (with the
+    reference's regularised pairs: the SnTe frequencies at 250 K hit both
+    branches), and L_linop / M_linop as the reference builds them
+    (DL:478-489), used the way its static solver does (DL:3935-3945)."""
+    import scipy.sparse.linalg as SLA
+    from oracle import lanczos_oracle as O
+    from tests.test_gpu_lanczos import _product_from_fixture
+    lanc = _product_from_fixture(snte, use_wigner=False)
+    rng = np.random.default_rng(23)
+    v = rng.normal(size=len(lanc.psi))
+    for transpose in (False, True):
+        got = lanc.apply_L1_inverse_FT(v, transpose=transpose)
+        ref = O.harmonic_L1_inverse(v, lanc.w, lanc.T, transpose=transpose)
+        assert rel(got, ref) < 1e-13
+    assert rel(lanc.M_linop.matvec(v), O.harmonic_L1_inverse(v, lanc.w, lanc.T)) < 1e-13
+    assert rel(lanc.M_linop.rmatvec(v),
+               O.harmonic_L1_inverse(v, lanc.w, lanc.T, transpose=True)) < 1e-13
+    assert rel(lanc.L_linop.matvec(v), lanc.apply_full_L(v.copy())) < 1e-14
+    assert rel(lanc.L_linop.rmatvec(v), lanc.apply_full_L(v.copy(), transpose=True)) < 1e-14
+    # SciPy drives them as operators (the consumers at DL:3935-3966 do)
+    B = rng.normal(size=(len(v), 2))
+    LB = lanc.L_linop.matmat(B)
+    for k in range(2):
+        assert rel(LB[:, k], lanc.apply_full_L(B[:, k].copy())) < 1e-14
+    assert isinstance(lanc.M_linop * lanc.L_linop, SLA.LinearOperator)
+    lanc.close()
+
+
 def test_stopped_chain_reports_one_b_fewer():
     """A chain that exhausts its Krylov space stops on |b| < tol with
     len(b) == len(a) - 1, whatever the device memory held before."""

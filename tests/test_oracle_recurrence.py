@@ -118,3 +118,30 @@ def test_tridiagonal_resolvent_and_ritz_values_on_a_dense_L():
     else:   # a smaller invariant space: every Ritz value is an eigenvalue of L
         for e in ev_T:
             assert np.min(np.abs(ev_L - e)) < 1e-10 * scale
+
+
+def test_harmonic_inverse_closed_form():
+    """The preconditioner (DL:2451-2549) inverts the standard-representation
+    harmonic L1 (DL:2287-2445) wherever the reference does not regularise it:
+    on R and on the pairs of distinct populated frequencies, M L1 = I, for
+    the plain and the transposed blocks."""
+    rng = np.random.default_rng(0)
+    n = 7
+    w = np.sort(rng.uniform(0.05, 0.5, n))
+    T = 1e6                         # every mode populated (0.01 w RY_TO_K < T)
+    psi = rng.normal(size=O.len_psi(n))
+    r, c = O.upper_pairs(n)
+    kept = np.concatenate([np.ones(n, bool), r < c, r < c])
+    for transpose in (False, True):
+        Lp = O.harmonic_L1(psi, w, T, False, transpose=transpose)
+        back = O.harmonic_L1_inverse(Lp, w, T, transpose=transpose)
+        assert np.max(np.abs(back - psi)[kept]) < 1e-9 * np.max(np.abs(psi))
+    # equal frequencies: the determinant vanishes and only -1/X (Y sector)
+    # and, when populated, -1/Y1 (ReA sector) remain, so an input on one
+    # sector of a diagonal pair maps back onto that sector alone
+    k = n                             # first diagonal pair (0, 0)
+    e = np.zeros(O.len_psi(n))
+    e[k] = 1.0
+    out = O.harmonic_L1_inverse(e, w, T)
+    nz = np.flatnonzero(out)
+    assert list(nz) == [k]
