@@ -36,6 +36,14 @@ constexpr int kTile = 128;      // column tile of the weighted outer-product GEM
 constexpr int kRowTile = 64;    // image rows per workgroup of the weights GEMM
 constexpr int kMaxChains = 8;   // perturbations applied in one batched call
 constexpr int kPartBlocks = 1024;  // blocks of the grid-wide maxima in the split prep
+// waves per work-group of the FP64 outer-product GEMM (4: 64 x 64 per wave,
+// two work-groups per CU; 8: 64 x 32 per wave, two work-groups and four
+// waves per SIMD, measured 17.6 ms against 22.6 ms at the 128-atom
+// workload); its launches use 64 * this many threads
+#ifndef TDS_F64_OUTER_WAVES
+#define TDS_F64_OUTER_WAVES 8
+#endif
+constexpr int kF64OuterThreads = 64 * TDS_F64_OUTER_WAVES;
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 inline long round_up_l(long x, long m) { return (x + m - 1) / m * m; }

@@ -17,9 +17,9 @@ namespace tdscha {
 
 __global__ void weights_f64(const double*, const double*, const double*, const double*,
                             const double*, const double*, const double*, int, long, int, long,
-                            long, long, double*);
+                            long, long, long, int, double*);
 __global__ void outer_f64(const double*, const double*, const double*, const double*, int, long,
-                          int, int, long, long, long, int, double*, double*);
+                          int, int, long, long, long, int, int, double*, double*);
 __global__ void reduce_ksplit(const double*, const double*, int, int, double*, double*, double);
 
 // 64 MiB, held by every device the library runs on until the process ends
@@ -261,10 +261,12 @@ int tdscha_perturb_averages_noalloc(int n_modes, int n_configs, const double* X,
     hipLaunchKernelGGL(na_expand, dim3((unsigned)Mc, (np + 127) / 128), dim3(128), 0, s, X, Y,
                        rho, syms, n_degeneracy, degenerate_space, sym_block_id, n_modes, np,
                        n_syms, n_blocks, dmax, b0, lo, hi, U, F, rimg);
-    hipLaunchKernelGGL(weights_f64, dim3((unsigned)(Mc / 64), 1), dim3(256), 0, s, U, F, Y1p, R1p,
-                       fy, fpsi, rimg, np, Mc, apply_d4, 0L, 0L, Mc, coef);
-    hipLaunchKernelGGL(outer_f64, dim3((unsigned)(ntile * ntile * L.ks), 1), dim3(256), 0, s, U,
-                       F, coef, fy, np, Mc, L.ks, ntile, 0L, Mc, rps, chunk > 0 ? 1 : 0, zpart,
+    // one chain: work-group id = row tile (Mc / 64 tiles)
+    hipLaunchKernelGGL(weights_f64, dim3((unsigned)(Mc / 64)), dim3(256), 0, s, U, F, Y1p, R1p,
+                       fy, fpsi, rimg, np, Mc, apply_d4, 0L, Mc / 64, 0L, Mc, 1, coef);
+    hipLaunchKernelGGL(outer_f64, dim3((unsigned)(ntile * ntile * L.ks)), dim3(kF64OuterThreads),
+                       0, s, U, F,
+                       coef, fy, np, Mc, L.ks, ntile, 0L, Mc, rps, chunk > 0 ? 1 : 0, 1, zpart,
                        fpart);
   }
   hipLaunchKernelGGL(reduce_ksplit, dim3((unsigned)((nn + 255) / 256), 1), dim3(256), 0, s, zpart,

@@ -57,34 +57,51 @@ __global__ void expand_images(const double* __restrict__ X, const double* __rest
 }
 
 // ---------------------------------------------------------------------------
-// weights: one workgroup = 64 image rows; column passes of 128; K = np
+// Work-group order of both GEMMs: the hardware deals work-groups to the 8
+// XCDs in turn (id % 8), so the chains of one tile get ids 8 apart and run on
+// one XCD at about the same time, sharing its L2 for the image rows.
 // ---------------------------------------------------------------------------
-constexpr int WA_KC = 16;
 
-__global__ __launch_bounds__(256) void weights_f64(
+// weights: one work-group = 64 image rows x a 128-column pass; K = np in
+// chunks of 16, double-buffered: the next chunk's global loads are in
+// flight while the MFMAs consume the current one (one barrier per chunk)
+constexpr int WA_KC = 16;
+// LDS row padding (doubles) of the K-chunk tiles: 16 puts the four 16-lane
+// groups of an MFMA operand read in distinct bank halves
+#ifndef TDS_F64_PAD
+#define TDS_F64_PAD 16
+#endif
+constexpr int kPad = TDS_F64_PAD;
+
+__global__ __launch_bounds__(256, 2) void weights_f64(
     const double* __restrict__ U, const double* __restrict__ F,
     const double* __restrict__ Y1all, const double* __restrict__ R1all,
     const double* __restrict__ fy, const double* __restrict__ fpsi,
-    const double* __restrict__ rho_img, int np, long Mp, int apply_d4, long tile0, long row_lo,
-    long row_hi, double* __restrict__ coef_all) {
-  const int chain = blockIdx.y;
+    const double* __restrict__ rho_img, int np, long Mp, int apply_d4, long tile0, long tile1,
+    long row_lo, long row_hi, int nc, double* __restrict__ coef_all) {
+  const int xcd = blockIdx.x & 7;
+  const int jj = blockIdx.x >> 3;
+  const int chain = jj % nc;
+  const long tile = tile0 + (long)(jj / nc) * 8 + xcd;
+  if (tile >= tile1) return;   // whole work-group, before any barrier
   const double* Y1 = Y1all + (long)chain * np * np;
   const double* R1 = R1all + (long)chain * np;
   double* coef = coef_all + (long)chain * Mp * 4;
-  const long r0 = (tile0 + blockIdx.x) * 64;
+  const long r0 = tile * 64;
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int lr = lane & 15;   // lane within a 16-group
   const int lg = lane >> 4;   // 16-group index
 
-  __shared__ double sU[64][WA_KC + 1];
-  __shared__ double sY[WA_KC][kTile + 2];
+  __shared__ double sU[2][64][WA_KC + 1];
+  __shared__ double sY[2][WA_KC][kTile + kPad];
   __shared__ double red[4][64][4];
   __shared__ double rowsum[64][4];
 
   rowsum[tid >> 2][tid & 3] = 0.0;
 
+  double ru[4], ry[8];
   for (int c0 = 0; c0 < np; c0 += kTile) {
     v4d acc[4][2];
 #pragma unroll
@@ -92,36 +109,54 @@ __global__ __launch_bounds__(256) void weights_f64(
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
 
-    for (int k0 = 0; k0 < np; k0 += WA_KC) {
-      __syncthreads();
-      // stage U[r0:r0+64, k0:k0+16]  (1024 values, 4 per thread)
+    // U[r0:r0+64, k0:k0+16] (4 per thread) and Y1[k0:k0+16, c0:c0+128] (8 per thread)
+    auto load = [&](int k0) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int idx = tid + 256 * e;
-        const int rr = idx >> 4, kk = idx & 15;
-        sU[rr][kk] = U[(r0 + rr) * np + k0 + kk];
+        ru[e] = U[(r0 + (idx >> 4)) * np + k0 + (idx & 15)];
       }
-      // stage Y1[k0:k0+16, c0:c0+128]  (2048 values, 8 per thread)
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int idx = tid + 256 * e;
-        const int kk = idx >> 7, cc = idx & 127;
-        sY[kk][cc] = Y1[(long)(k0 + kk) * np + c0 + cc];
+        ry[e] = Y1[(long)(k0 + (idx >> 7)) * np + c0 + (idx & 127)];
       }
-      __syncthreads();
+    };
+    auto store = [&](int b) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int idx = tid + 256 * e;
+        sU[b][idx >> 4][idx & 15] = ru[e];
+      }
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int idx = tid + 256 * e;
+        sY[b][idx >> 7][idx & 127] = ry[e];
+      }
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    int cur = 0;
+    for (int k0 = 0; k0 < np; k0 += WA_KC) {
+      const bool more = k0 + WA_KC < np;
+      if (more) load(k0 + WA_KC);
 #pragma unroll
       for (int kk = 0; kk < WA_KC; kk += 4) {
         double a[4], b[2];
 #pragma unroll
-        for (int rb = 0; rb < 4; ++rb) a[rb] = sU[16 * rb + lr][kk + lg];
+        for (int rb = 0; rb < 4; ++rb) a[rb] = sU[cur][16 * rb + lr][kk + lg];
 #pragma unroll
-        for (int cb = 0; cb < 2; ++cb) b[cb] = sY[kk + lg][32 * wave + 16 * cb + lr];
+        for (int cb = 0; cb < 2; ++cb) b[cb] = sY[cur][kk + lg][32 * wave + 16 * cb + lr];
 #pragma unroll
         for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
           for (int cb = 0; cb < 2; ++cb)
             acc[rb][cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rb], b[cb], acc[rb][cb], 0, 0, 0);
       }
+      if (more) store(cur ^ 1);
+      __syncthreads();
+      cur ^= 1;
     }
 
     // fused epilogue: partial row dots over this wave's 32 columns
@@ -190,23 +225,33 @@ __global__ __launch_bounds__(256) void weights_f64(
 }
 
 // ---------------------------------------------------------------------------
-// outer: split-K Z[mu, nu] = sum_b U[b, mu] B[b, nu] over a range of images
+// outer: split-K Z[mu, nu] = sum_b U[b, mu] B[b, nu] over a range of images,
+// 128 x 128 tiles, 4 waves of 64 x 64; K in chunks of 8 images,
+// double-buffered like the weights GEMM (raw U, F and coefficients of the
+// next chunk in registers during the MFMAs, B built when they are stored).
+// Two work-groups per CU: one's barrier overlaps the other's MFMAs.
 // ---------------------------------------------------------------------------
-constexpr int WB_KC = 16;
+constexpr int WB_KC = 8;
+constexpr int kOW = TDS_F64_OUTER_WAVES;   // 4 (64 x 64 per wave) or 8 (64 x 32 per wave)
+constexpr int kOT = kF64OuterThreads;      // threads
+constexpr int kWN = kOW == 4 ? 64 : 32;    // columns per wave
+constexpr int kNJ = kWN / 16;              // 16-column blocks per wave
+constexpr int kRS = kOT / kTile;           // image rows staged per pass of the threads
+constexpr int kRE = WB_KC / kRS;           // rows per thread and chunk
 
-__global__ __launch_bounds__(256) void outer_f64(
+__global__ __launch_bounds__(kOT, kOW == 8 ? 4 : 2) void outer_f64(
     const double* __restrict__ U, const double* __restrict__ F,
     const double* __restrict__ coef_all, const double* __restrict__ fy, int np, long Mp,
-    int ksplit, int ntile, long kbase, long kend, long rows_per_split, int accumulate,
+    int ksplit, int ntile, long kbase, long kend, long rows_per_split, int accumulate, int nc,
     double* __restrict__ zpart_all, double* __restrict__ fpart_all) {
-  const int id = blockIdx.x;
-  const int xcd = id & 7;
-  const int slot = id >> 3;
+  const int xcd = blockIdx.x & 7;
+  const int jj = blockIdx.x >> 3;
+  const int chain = jj % nc;
+  const int rest = jj / nc;
   const int t2 = ntile * ntile;
-  const int ks = xcd + 8 * (slot / t2);
-  const int tile = slot % t2;
+  const int tile = rest % t2;
+  const int ks = xcd + 8 * (rest / t2);
   const int tm = tile / ntile, tn = tile % ntile;
-  const int chain = blockIdx.y;
   const double* coef = coef_all + (long)chain * Mp * 4;
   double* zpart = zpart_all + ((long)chain * ksplit + ks) * np * np;
   double* fpart = fpart_all + ((long)chain * ksplit + ks) * np;
@@ -217,65 +262,93 @@ __global__ __launch_bounds__(256) void outer_f64(
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int lr = lane & 15, lg = lane >> 4;
-  const int wm = wave >> 1, wn = wave & 1;
+  const int wm = wave / (kTile / kWN), wn = wave % (kTile / kWN);   // 64 rows x kWN columns
 
-  __shared__ double sA[WB_KC][kTile + 2];
-  __shared__ double sB[WB_KC][kTile + 2];
-  __shared__ double sF[2][kTile];
+  __shared__ double sA[2][WB_KC][kTile + kPad];
+  __shared__ double sB[2][WB_KC][kTile + kPad];
+  __shared__ double sF[kRS][kTile];
 
-  v4d acc[4][4];
+  v4d acc[4][kNJ];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < kNJ; ++b) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
 
   const long kb = kbase + (long)ks * rows_per_split;
   long ke = kb + rows_per_split;
   if (ke > kend) ke = kend;
   const int col = tid & 127;
-  const int rsub = tid >> 7;  // 0 or 1
+  const int rsub = tid >> 7;  // rows rsub, rsub + kRS, ... of a chunk
   const double fyn = fy[nu0 + col];
   double fl = 0.0;
-
-  for (long k0 = kb; k0 < ke; k0 += WB_KC) {
-    __syncthreads();
+  // next chunk's raw values; on diagonal tiles U of the nu columns is the
+  // A value itself, and rx, ry carry alpha, beta instead
+  double rc1[kRE], rc2[kRE], rum[kRE], rfn[kRE], rx[kRE], ry[kRE];
+  const long rstride = (long)kRS * np;       // between a thread's rows
+  auto load = [&](long k0) {
+    const long row0 = k0 + rsub;
+    const double* pu = U + row0 * np + col;
+    const double* pf = F + row0 * np + nu0 + col;
+    const double* pc = coef + row0 * 4;
 #pragma unroll
-    for (int e = 0; e < WB_KC / 2; ++e) {
-      const int kk = rsub + 2 * e;
-      const long row = k0 + kk;
-      const double* cf = coef + row * 4;
-      const double c1 = cf[0], c2 = cf[1];
-      const double um = U[row * np + mu0 + col];
-      const double un = diag ? um : U[row * np + nu0 + col];
-      const double fn = F[row * np + nu0 + col];
-      sA[kk][col] = um;
-      sB[kk][col] = c1 * fn + 0.5 * c2 * fyn * un;
-      if (diag) fl += cf[2] * fn + cf[3] * fyn * un;
+    for (int e = 0; e < kRE; ++e) {
+      rc1[e] = pc[4 * kRS * e];
+      rc2[e] = pc[4 * kRS * e + 1];
+      rum[e] = pu[e * rstride + mu0];
+      rfn[e] = pf[e * rstride];
+      if (diag) {
+        rx[e] = pc[4 * kRS * e + 2];
+        ry[e] = pc[4 * kRS * e + 3];
+      } else {
+        rx[e] = pu[e * rstride + nu0];
+      }
     }
-    __syncthreads();
+  };
+  auto store = [&](int b) {
+#pragma unroll
+    for (int e = 0; e < kRE; ++e) {
+      const int kk = rsub + kRS * e;
+      const double un = diag ? rum[e] : rx[e];
+      sA[b][kk][col] = rum[e];
+      sB[b][kk][col] = rc1[e] * rfn[e] + 0.5 * rc2[e] * fyn * un;
+      if (diag) fl += rx[e] * rfn[e] + ry[e] * fyn * un;
+    }
+  };
+  if (kb < ke) {
+    load(kb);
+    store(0);
+  }
+  __syncthreads();
+  int cur = 0;
+  for (long k0 = kb; k0 < ke; k0 += WB_KC) {
+    const bool more = k0 + WB_KC < ke;
+    if (more) load(k0 + WB_KC);
 #pragma unroll
     for (int kk = 0; kk < WB_KC; kk += 4) {
-      double a[4], b[4];
+      double a[4], b[kNJ];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = sA[kk + lg][64 * wm + 16 * i + lr];
+      for (int i = 0; i < 4; ++i) a[i] = sA[cur][kk + lg][64 * wm + 16 * i + lr];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) b[j] = sB[kk + lg][64 * wn + 16 * j + lr];
+      for (int j = 0; j < kNJ; ++j) b[j] = sB[cur][kk + lg][kWN * wn + 16 * j + lr];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < kNJ; ++j)
           acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    if (more) store(cur ^ 1);
+    __syncthreads();
+    cur ^= 1;
   }
   // write the partial tile
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < kNJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int mu = mu0 + 64 * wm + 16 * i + 4 * r + lg;
-        const int nu = nu0 + 64 * wn + 16 * j + lr;
+        const int nu = nu0 + kWN * wn + 16 * j + lr;
         // accumulate: add to the partial of earlier image chunks (each
         // element has one writer, chunks run in order)
         const double prev = accumulate ? zpart[(long)mu * np + nu] : 0.0;
@@ -285,8 +358,11 @@ __global__ __launch_bounds__(256) void outer_f64(
     sF[rsub][col] = fl;
     __syncthreads();
     if (tid < kTile) {
+      double sum = 0.0;
+#pragma unroll
+      for (int q = 0; q < kRS; ++q) sum += sF[q][tid];
       const double prev = accumulate ? fpart[nu0 + tid] : 0.0;
-      fpart[nu0 + tid] = prev + (sF[0][tid] + sF[1][tid]);
+      fpart[nu0 + tid] = prev + sum;
     }
   }
 }
@@ -328,17 +404,19 @@ int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* ev) 
   // (rows outside the range get zero coefficients), and the outer GEMM's K
   // range from the 16-row step below row_lo to the step above row_hi
   const long t0 = c->row_lo / 64, t1 = (c->row_hi + 63) / 64;
-  dim3 ga((unsigned)(t1 - t0), nc);
+  // tiles dealt 8 at a time (one per XCD), every chain of a tile on its XCD
+  const long ga = round_up_l(t1 - t0, 8) * nc;
   if (ev) (void)hipEventRecord(ev[0], s);
-  hipLaunchKernelGGL(weights_f64, ga, dim3(256), 0, s, c->U, c->F, c->Y1, c->R1, c->fy, c->fpsi,
-                     c->rho_img, c->np, c->Mp, apply_d4, t0, c->row_lo, c->row_hi, c->coef);
+  hipLaunchKernelGGL(weights_f64, dim3((unsigned)ga), dim3(256), 0, s, c->U, c->F, c->Y1, c->R1,
+                     c->fy, c->fpsi, c->rho_img, c->np, c->Mp, apply_d4, t0, t1, c->row_lo,
+                     c->row_hi, nc, c->coef);
   if (ev) (void)hipEventRecord(ev[1], s);
   const long kb = c->row_lo / WB_KC * WB_KC;
   const long ke = round_up_l(c->row_hi, WB_KC);
   const long rps = round_up_l((ke - kb + c->ksplit - 1) / c->ksplit, WB_KC);
-  dim3 gb((unsigned)(ntile * ntile * c->ksplit), nc);
-  hipLaunchKernelGGL(outer_f64, gb, dim3(256), 0, s, c->U, c->F, c->coef, c->fy, c->np, c->Mp,
-                     c->ksplit, ntile, kb, ke, rps, 0, c->zpart, c->fpart);
+  const long gb = (long)ntile * ntile * c->ksplit * nc;
+  hipLaunchKernelGGL(outer_f64, dim3((unsigned)gb), dim3(kOT), 0, s, c->U, c->F, c->coef, c->fy,
+                     c->np, c->Mp, c->ksplit, ntile, kb, ke, rps, 0, nc, c->zpart, c->fpart);
   if (ev) (void)hipEventRecord(ev[2], s);
   const long nn = (long)c->np * c->np;
   const double norm = 1.0 / (c->n_eff * c->n_syms);

@@ -103,16 +103,25 @@ def test_projection_with_a_permutation_set_that_is_not_a_group(snte):
 
 
 def test_run_FT_reproduces_reference_golden_run(snte):
-    """Biconjugate standard-representation run of test_julia_lf.py:24-45."""
+    """Biconjugate standard-representation run of test_julia_lf.py:24-45.
+
+    The tenth b*c (3e-18, the product of two coefficients of order 1e-9 at
+    the end of an almost exhausted Krylov space) is ill-conditioned: in the
+    oracle's FP64 recurrence a relative perturbation of 1e-16 of every L
+    application moves it by 7e-8, of 4e-16 by 1.1e-7, while the other nine
+    move by at most 2.5e-13.  It is held to the reference's own tolerance on
+    these coefficients (rtol 1e-6, test_julia_lf.py:45), the rest to 1e-10.
+    """
     lanc = _product_from_fixture(snte, use_wigner=False)
     lanc.prepare_mode(10)
     lanc.run_FT(10, verbose=False)
     a = np.array(lanc.a_coeffs)
     bc = np.array(lanc.b_coeffs) * np.array(lanc.c_coeffs)
-    print("a", np.max(np.abs(a / snte["a_coeffs"] - 1)),
-          "bc", np.max(np.abs(bc / (snte["b_coeffs"] * snte["c_coeffs"]) - 1)))
+    dbc = np.abs(bc / (snte["b_coeffs"] * snte["c_coeffs"]) - 1)
+    print("a", np.max(np.abs(a / snte["a_coeffs"] - 1)), "bc", dbc)
     assert np.max(np.abs(a / snte["a_coeffs"] - 1)) < 1e-10
-    assert np.max(np.abs(bc / (snte["b_coeffs"] * snte["c_coeffs"]) - 1)) < 1e-7
+    assert np.max(dbc[:9]) < 1e-10
+    assert dbc[9] < 1e-6
     assert abs(lanc.c_coeffs[4] - 6.59744344e-07) / 6.59744344e-07 < 1e-6
     lanc.close()
 
