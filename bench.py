@@ -343,6 +343,29 @@ def main():
                                                         precision=prec)[0]
                 log("%s: %d-step chains for the accuracy gate (untimed, %.1fs)" % (
                     prec, gate_steps, time.perf_counter() - t0))
+        # the FP64 chains again with another K split of the outer GEMM (same
+        # arithmetic, another summation order): how far FP64 itself carries
+        # the coefficients of this workload
+        ks0 = lanc.engine.ksplit
+        lanc.engine.set_ksplit(max(8, (ks0 // 2) // 8 * 8))
+        fp64_alt = lanc.run_FT_batched(psis, gate_steps, reorth=True, precision="fp64")[0]
+        lanc.engine.set_ksplit(ks0)
+
+        def parting_step(ch1, ch0, thr=1e-6):
+            """First step whose a or b differs by more than thr (relative)."""
+            first = gate_steps
+            for (a1, b1), (a0, b0) in zip(ch1, ch0):
+                k = min(len(a1), len(a0), len(b1), len(b0))
+                d = np.maximum(np.abs(a1[:k] - a0[:k]) / np.abs(a0[:k]),
+                               np.abs(b1[:k] - b0[:k]) / np.abs(b0[:k]))
+                bad = np.flatnonzero(d > thr)
+                first = min(first, int(bad[0]) if len(bad) else k)
+            return first
+
+        fp64_parting = parting_step(fp64_alt, long_chains["fp64"])
+        fp64_spec = spec_error(fp64_alt, long_chains["fp64"])[0]
+        log("fp64 vs fp64 with K split %d instead of %d: coefficients part (1e-6) at step %d, "
+            "spectrum %.2e" % (max(8, (ks0 // 2) // 8 * 8), ks0, fp64_parting, fp64_spec))
         lanc.precision = "fp64"
         L64 = lanc.apply_L_batch(psis)
         for prec in results:
@@ -364,11 +387,15 @@ def main():
                          "spectrum_linf_rel_err_timed_chains": short[prec],
                          "timed_chain_steps": K,
                          "coeff_a_max_rel_err_first50": ea, "coeff_b_max_rel_err_first50": eb,
+                         "coeff_parting_step_1e-6": parting_step(long_chains[prec],
+                                                                 long_chains["fp64"]),
+                         "fp64_vs_fp64_other_ksplit_parting_step_1e-6": fp64_parting,
+                         "fp64_vs_fp64_other_ksplit_spectrum_linf_rel_err": fp64_spec,
                          "Lv_rel_l2_err": lv}
             log("%s vs FP64 on %d-step chains: spectrum %.2e (per chain %s), a %.2e, b %.2e "
-                "(first 50), L.v %.2e; %d-step timed chains: spectrum %.2e" % (
-                    prec, gate_steps, se, ["%.2e" % x for x in errs], ea, eb, lv, K,
-                    short[prec]))
+                "(first 50), coefficients part at step %d, L.v %.2e; %d-step timed chains: "
+                "spectrum %.2e" % (prec, gate_steps, se, ["%.2e" % x for x in errs], ea, eb,
+                                   acc[prec]["coeff_parting_step_1e-6"], lv, K, short[prec]))
         spec_err = acc[args.precision]["spectrum_linf_rel_err"]
     if "bf16x2" in results and args.precision != "bf16x2":
         r = results["bf16x2"]

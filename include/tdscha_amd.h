@@ -196,6 +196,24 @@ int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_
                        double* elapsed_ms);
 
 /*
+ * Device-resident biconjugate Lanczos (run_FT without run_simm,
+ * Modules/DynamicalLanczos.py:6126-6240, n_rep_orth = 0): right vectors q
+ * and left vectors p with plain dot products, L q and L^T p per step
+ * (transpose semantics of tdscha_apply_L, in the standard or the Wigner
+ * representation), the reference's a, b, c and s_norm bookkeeping.  Chains
+ * stop when |b| or |c| < tol (a kept, b and c dropped) or at caps[c] steps
+ * (caps may be NULL).  reverse / shift as tdscha_apply_L.  One step is
+ * captured as a hipGraph and replayed with use_graph.  a_out, b_out, c_out
+ * [n_chains, n_steps]; n_done[c] = number of a coefficients.
+ */
+int tdscha_lanczos_run_biconjugate(tdscha_ctx* ctx, int n_chains, const double* psi0,
+                                   int n_steps, const int* caps, double tol, int use_wigner,
+                                   int precision, int ignore_harmonic, int ignore_v3,
+                                   int ignore_v4, int reverse, double shift, int use_graph,
+                                   double* a_out, double* b_out, double* c_out, int* n_done,
+                                   double* elapsed_ms);
+
+/*
  * Continued-fraction Green function with the standard terminator
  * (Modules/DynamicalLanczos.py:5032-5162) on n_w frequencies.  a [n_a],
  * b [n_b] (n_b >= n_a - 1), c [n_b] or NULL (c = b).  gf_out [n_w, 2] holds

@@ -50,6 +50,10 @@ def _signatures(L):
     L.tdscha_lanczos_run.argtypes = [_vp, C.c_int, _vp, C.c_int, _vp, C.c_double, C.c_int,
                                      C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp,
                                      _vp, _vp]
+    L.tdscha_lanczos_run_biconjugate.argtypes = [_vp, C.c_int, _vp, C.c_int, _vp, C.c_double,
+                                                 C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                                 C.c_int, C.c_double, C.c_int, _vp, _vp, _vp,
+                                                 _vp, _vp]
     L.tdscha_continued_fraction.argtypes = [C.c_int, _vp, C.c_int, _vp, _vp, C.c_int, _vp,
                                             C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                             C.c_int, C.c_double, C.c_double, _vp]

@@ -351,11 +351,319 @@ __global__ void fill_zero(double* p, long n) {
   if (i < n) p[i] = 0.0;
 }
 
+// ---------------------------------------------------------------------------
+// Biconjugate Lanczos (run_FT without run_simm, DL:6126-6240, n_rep_orth = 0):
+// left and right vectors p, q with plain dot products, two L applications per
+// step (L q and L^T p).  Per chain, scal holds
+//   [0] c_(k-1)  [1] c_(k-2)  [2] s_norm_k  [3] s_norm_(k-1)  [4] b_(k-1)
+// ---------------------------------------------------------------------------
+struct BiState {
+  int nc = 0, kmax = 0;
+  long len = 0;
+  double *q = nullptr, *qprev = nullptr, *p = nullptr, *pprev = nullptr;  // [nc, len]
+  double *Lq = nullptr, *pL = nullptr, *r = nullptr, *s = nullptr;        // [nc, len]
+  double* ones = nullptr;   // [len]
+  double* part = nullptr;   // [nc, 4, kRedBlocks]: <p, Lq>, <s, s>, <r, r>, <s, r>
+  double *a = nullptr, *b = nullptr, *c = nullptr;   // [nc, kmax]
+  double* scal = nullptr;   // [nc, 8]
+  int *k = nullptr, *active = nullptr, *cap = nullptr, *ndone = nullptr;
+};
+
+static void free_bi(BiState& s) {
+  void* p[] = {s.q, s.qprev, s.p, s.pprev, s.Lq, s.pL, s.r, s.s, s.ones, s.part, s.a, s.b, s.c,
+               s.scal, s.k, s.active, s.cap, s.ndone};
+  for (void* x : p)
+    if (x) (void)hipFree(x);
+}
+
+__global__ void fill_value(double* p, long n, double v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// q = p = psi0 / |psi0| (plain norm); s_norm_0 = 1
+__global__ void bi_init(const double* __restrict__ part, long ps, const double* __restrict__ psi0,
+                        long len, double* __restrict__ q, double* __restrict__ p,
+                        double* __restrict__ scal) {
+  const int c = blockIdx.y;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x < 8) scal[c * 8 + threadIdx.x] = threadIdx.x == 2 ? 1.0 : 0.0;
+  if (i >= len) return;
+  const double v = psi0[c * len + i] / sqrt(sum_partials(part + c * ps));
+  q[c * len + i] = v;
+  p[c * len + i] = v;
+}
+
+// a = <p, Lq> p_norm;  r = Lq - a q - c_(k-1) q_prev;
+// s = pL - a p - b_(k-1) p_prev (old_p_norm / p_norm)
+__global__ void bi_residual(const double* __restrict__ part, long ps,
+                            const double* __restrict__ q, const double* __restrict__ qprev,
+                            const double* __restrict__ p, const double* __restrict__ pprev,
+                            const double* __restrict__ Lq, const double* __restrict__ pL, long len,
+                            int kmax, const int* __restrict__ kk, const int* __restrict__ active,
+                            const double* __restrict__ scal, double* __restrict__ a_out,
+                            double* __restrict__ r, double* __restrict__ s) {
+  const int c = blockIdx.y;
+  if (!active[c]) return;
+  const int k = kk[c];
+  const double* sc = scal + c * 8;
+  const double c_old = k > 0 ? sc[0] : 1.0;
+  const double p_norm = sc[2] / c_old;
+  const double ak = sum_partials(part + c * ps) * p_norm;
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) a_out[c * kmax + k] = ak;
+  if (i >= len) return;
+  const long o = (long)c * len + i;
+  double rv = Lq[o] - ak * q[o];
+  double sv = pL[o] - ak * p[o];
+  if (k > 0) {
+    rv -= sc[0] * qprev[o];
+    const double old_p_norm = k < 2 ? sc[3] : sc[3] / sc[1];
+    sv -= sc[4] * pprev[o] * (old_p_norm / p_norm);
+  }
+  r[o] = rv;
+  s[o] = sv;
+}
+
+// <s, s>, <r, r>, <s, r> partial sums into slots 1..3
+__global__ void bi_dots(const double* __restrict__ r, const double* __restrict__ s, long len,
+                        const int* __restrict__ active, double* __restrict__ part, long ps) {
+  const int c = blockIdx.y;
+  if (!active[c]) return;
+  const double* rc = r + (long)c * len;
+  const double* scv = s + (long)c * len;
+  double ss = 0.0, rr = 0.0, sr = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < len;
+       i += (long)gridDim.x * blockDim.x) {
+    const double x = scv[i], y = rc[i];
+    ss += x * x;
+    rr += y * y;
+    sr += x * y;
+  }
+  __shared__ double red[3][kRedThreads];
+  red[0][threadIdx.x] = ss;
+  red[1][threadIdx.x] = rr;
+  red[2][threadIdx.x] = sr;
+  __syncthreads();
+  for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int j = 0; j < 3; ++j) red[j][threadIdx.x] += red[j][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[c * ps + (long)(1 + threadIdx.x) * kRedBlocks + blockIdx.x] = red[threadIdx.x][0];
+}
+
+// b = |r|, s_raw = |s|, c = <s / s_raw, r / b> s_raw p_norm; the chain stops
+// (a kept, b and c not) when |b| or |c| is below tol, or at its cap
+__device__ __forceinline__ void bi_scalars(const double* part, const double* sc, int k,
+                                           double* b, double* s_raw, double* cc,
+                                           double* p_norm) {
+  const double c_old = k > 0 ? sc[0] : 1.0;
+  *p_norm = sc[2] / c_old;
+  *s_raw = sqrt(sum_partials(part + 1 * kRedBlocks));
+  *b = sqrt(sum_partials(part + 2 * kRedBlocks));
+  const double sr = sum_partials(part + 3 * kRedBlocks);
+  *cc = (sr / (*s_raw * *b)) * (*s_raw * *p_norm);
+}
+
+__global__ void bi_advance(const double* __restrict__ part, long ps, long len, double tol,
+                           const int* __restrict__ kk, const int* __restrict__ active,
+                           const int* __restrict__ cap, const double* __restrict__ scal,
+                           const double* __restrict__ r, const double* __restrict__ s,
+                           double* __restrict__ q, double* __restrict__ qprev,
+                           double* __restrict__ p, double* __restrict__ pprev) {
+  const int c = blockIdx.y;
+  if (!active[c]) return;
+  const int k = kk[c];
+  double b, s_raw, cc, p_norm;
+  bi_scalars(part + c * ps, scal + c * 8, k, &b, &s_raw, &cc, &p_norm);
+  const bool stop = !(fabs(b) >= tol) || !(fabs(cc) >= tol) || (k + 1 >= cap[c]);
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (stop || i >= len) return;
+  const long o = (long)c * len + i;
+  qprev[o] = q[o];
+  q[o] = r[o] / b;
+  pprev[o] = p[o];
+  p[o] = s[o] / s_raw;
+}
+
+__global__ void bi_commit(const double* __restrict__ part, long ps, int kmax, double tol,
+                          int* __restrict__ kk, int* __restrict__ active,
+                          const int* __restrict__ cap, int* __restrict__ ndone,
+                          double* __restrict__ scal, double* __restrict__ b_out,
+                          double* __restrict__ c_out) {
+  const int c = threadIdx.x;
+  if (!active[c]) return;
+  const int k = kk[c];
+  double* sc = scal + c * 8;
+  double b, s_raw, cc, p_norm;
+  bi_scalars(part + c * ps, sc, k, &b, &s_raw, &cc, &p_norm);
+  const bool small = !(fabs(b) >= tol) || !(fabs(cc) >= tol);
+  ndone[c] = k + 1;
+  if (!small) {
+    b_out[c * kmax + k] = b;
+    c_out[c * kmax + k] = cc;
+    sc[1] = sc[0];
+    sc[0] = cc;
+    sc[3] = sc[2];
+    sc[2] = s_raw * p_norm;
+    sc[4] = b;
+  }
+  if (small || k + 1 >= cap[c])
+    active[c] = 0;
+  else
+    kk[c] = k + 1;
+}
+
+static void bi_step(Ctx* c, BiState& st, int use_wigner, int precision, int ignore_harmonic,
+                    int ignore_v3, int ignore_v4, int reverse, double shift, double tol,
+                    hipStream_t s) {
+  const long len = st.len;
+  const int nc = st.nc;
+  const long ps = 4L * kRedBlocks;
+  const dim3 gv((unsigned)((len + 255) / 256), nc);
+  const dim3 gr(kRedBlocks, nc);
+  apply_L_dev(c, nc, st.q, st.Lq, use_wigner, 0, ignore_harmonic, ignore_v3, ignore_v4, reverse,
+              shift, precision, s);
+  apply_L_dev(c, nc, st.p, st.pL, use_wigner, 1, ignore_harmonic, ignore_v3, ignore_v4, reverse,
+              shift, precision, s);
+  hipLaunchKernelGGL(dot_partial, gr, dim3(kRedThreads), 0, s, st.p, len, st.Lq, len, st.ones,
+                     len, st.part, ps, 0);
+  hipLaunchKernelGGL(bi_residual, gv, dim3(256), 0, s, st.part, ps, st.q, st.qprev, st.p,
+                     st.pprev, st.Lq, st.pL, len, st.kmax, st.k, st.active, st.scal, st.a, st.r,
+                     st.s);
+  hipLaunchKernelGGL(bi_dots, gr, dim3(kRedThreads), 0, s, st.r, st.s, len, st.active, st.part,
+                     ps);
+  hipLaunchKernelGGL(bi_advance, gv, dim3(256), 0, s, st.part, ps, len, tol, st.k, st.active,
+                     st.cap, st.scal, st.r, st.s, st.q, st.qprev, st.p, st.pprev);
+  hipLaunchKernelGGL(bi_commit, dim3(1), dim3(nc), 0, s, st.part, ps, st.kmax, tol, st.k,
+                     st.active, st.cap, st.ndone, st.scal, st.b, st.c);
+}
+
 }  // namespace tdscha
 
 using namespace tdscha;
 
 extern "C" {
+
+int tdscha_lanczos_run_biconjugate(tdscha_ctx* ctx, int n_chains, const double* psi0,
+                                   int n_steps, const int* caps, double tol, int use_wigner,
+                                   int precision, int ignore_harmonic, int ignore_v3,
+                                   int ignore_v4, int reverse, double shift, int use_graph,
+                                   double* a_out, double* b_out, double* c_out, int* n_done,
+                                   double* elapsed_ms) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_lanczos_run_biconjugate: NULL context");
+  TDS_REQUIRE(n_chains >= 1 && n_chains <= kMaxChains, 2, "n_chains must be in [1, 8]");
+  TDS_REQUIRE(n_steps >= 1 && n_steps <= 100000, 2, "n_steps must be in [1, 100000]");
+  TDS_REQUIRE(psi0 && a_out && b_out && c_out && n_done, 2,
+              "tdscha_lanczos_run_biconjugate: NULL pointer");
+  TDS_REQUIRE(precision >= 0 && precision <= 2, 2, "precision must be 0, 1 or 2");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  int rc = setup_pairs(c);
+  if (rc) return rc;
+  if (precision != PREC_FP64) {
+    rc = ensure_split(c, precision);  // allocations must precede graph capture
+    if (rc) return rc;
+  }
+  const int n = c->n;
+  BiState st;
+  st.nc = n_chains;
+  st.kmax = n_steps;
+  st.len = n + (long)n * (n + 1);
+  const long len = st.len;
+  const size_t vb = sizeof(double) * n_chains * len;
+  bool ok = true;
+  double** vecs[] = {&st.q, &st.qprev, &st.p, &st.pprev, &st.Lq, &st.pL, &st.r, &st.s};
+  for (double** v : vecs) ok = ok && hipMalloc((void**)v, vb) == hipSuccess;
+  ok = ok && hipMalloc((void**)&st.ones, sizeof(double) * len) == hipSuccess &&
+       hipMalloc((void**)&st.part, sizeof(double) * n_chains * 4 * kRedBlocks) == hipSuccess &&
+       hipMalloc((void**)&st.a, sizeof(double) * n_chains * n_steps) == hipSuccess &&
+       hipMalloc((void**)&st.b, sizeof(double) * n_chains * n_steps) == hipSuccess &&
+       hipMalloc((void**)&st.c, sizeof(double) * n_chains * n_steps) == hipSuccess &&
+       hipMalloc((void**)&st.scal, sizeof(double) * n_chains * 8) == hipSuccess &&
+       hipMalloc((void**)&st.k, sizeof(int) * n_chains) == hipSuccess &&
+       hipMalloc((void**)&st.active, sizeof(int) * n_chains) == hipSuccess &&
+       hipMalloc((void**)&st.cap, sizeof(int) * n_chains) == hipSuccess &&
+       hipMalloc((void**)&st.ndone, sizeof(int) * n_chains) == hipSuccess;
+  if (!ok) {
+    free_bi(st);
+    set_error("tdscha_lanczos_run_biconjugate: device allocation failed");
+    return -1;
+  }
+  hipStream_t s = c->stream;
+  std::vector<int> hcap(n_chains, n_steps);
+  if (caps)
+    for (int i = 0; i < n_chains; ++i) hcap[i] = std::max(1, std::min(caps[i], n_steps));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  auto fail = [&](const char* what) {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    free_bi(st);
+    set_error(std::string("tdscha_lanczos_run_biconjugate: ") + what);
+    return -1;
+  };
+  if (hipMemcpyAsync(st.cap, hcap.data(), sizeof(int) * n_chains, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(st.Lq, psi0, vb, hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail("upload failed");
+  const long na = (long)n_chains * n_steps;
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, st.k, n_chains, 0);
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, st.active, n_chains, 1);
+  hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, st.ndone, n_chains, 0);
+  hipLaunchKernelGGL(fill_value, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, st.ones,
+                     len, 1.0);
+  hipLaunchKernelGGL(fill_zero, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, st.a, na);
+  hipLaunchKernelGGL(fill_zero, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, st.b, na);
+  hipLaunchKernelGGL(fill_zero, dim3((unsigned)((na + 255) / 256)), dim3(256), 0, s, st.c, na);
+  const long ps = 4L * kRedBlocks;
+  // |psi0|^2 into slot 0, then q = p = psi0 / |psi0| (psi0 staged in Lq)
+  hipLaunchKernelGGL(dot_partial, dim3(kRedBlocks, n_chains), dim3(kRedThreads), 0, s, st.Lq, len,
+                     st.Lq, len, st.ones, len, st.part, ps, 0);
+  hipLaunchKernelGGL(bi_init, dim3((unsigned)((len + 255) / 256), n_chains), dim3(256), 0, s,
+                     st.part, ps, st.Lq, len, st.q, st.p, st.scal);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+    return fail("initialisation failed");
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    return fail("event creation failed");
+  if (use_graph) {
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return fail("stream capture failed to start");
+    bi_step(c, st, use_wigner, precision, ignore_harmonic, ignore_v3, ignore_v4, reverse, shift,
+            tol, s);
+    if (hipStreamEndCapture(s, &graph) != hipSuccess) return fail("stream capture failed");
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+      return fail("graph instantiation failed");
+  }
+  (void)hipEventRecord(e0, s);
+  for (int step = 0; step < n_steps; ++step) {
+    if (use_graph) {
+      if (hipGraphLaunch(exec, s) != hipSuccess) return fail("graph launch failed");
+    } else {
+      bi_step(c, st, use_wigner, precision, ignore_harmonic, ignore_v3, ignore_v4, reverse,
+              shift, tol, s);
+    }
+  }
+  (void)hipEventRecord(e1, s);
+  if (hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+    return fail("step execution failed");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  if (elapsed_ms) *elapsed_ms = ms;
+  ok = hipMemcpy(a_out, st.a, sizeof(double) * na, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(b_out, st.b, sizeof(double) * na, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(c_out, st.c, sizeof(double) * na, hipMemcpyDeviceToHost) == hipSuccess &&
+       hipMemcpy(n_done, st.ndone, sizeof(int) * n_chains, hipMemcpyDeviceToHost) == hipSuccess;
+  if (!ok) return fail("download failed");
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  free_bi(st);
+  return 0;
+}
 
 int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_steps,
                        const int* caps, double tol, int reorth, int precision,

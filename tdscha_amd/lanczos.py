@@ -342,8 +342,16 @@ class Lanczos(object):
         return mask
 
     def run_FT(self, n_iter, save_dir=None, save_each=5, verbose=True, n_rep_orth=0, n_ortho=10,
-               flush_output=True, debug=False, prefix="LANCZOS", run_simm=None, optimized=False):
-        """DL:5958-6491: the reference recurrence around device L applications."""
+               flush_output=True, debug=False, prefix="LANCZOS", run_simm=None, optimized=False,
+               device=False):
+        """DL:5958-6491: the reference recurrence around device L applications.
+
+        ``device=True`` runs the same recurrence (biconjugate, or symmetric
+        with run_simm, n_rep_orth = 0) entirely on the device for a fresh
+        chain: only the coefficients come back (a_coeffs, b_coeffs,
+        c_coeffs), the Krylov bases are not kept."""
+        if device:
+            return self._run_FT_device(n_iter, run_simm, verbose)
         self.verbose = verbose
         if not self.initialized:
             self.prepare_symmetrization()
@@ -453,6 +461,54 @@ class Lanczos(object):
             if save_dir is not None and (i + 1) % save_each == 0:
                 self.save_status("%s/%s_STEP%d" % (save_dir, prefix, i + 1))
         return converged
+
+    def _run_FT_device(self, n_iter, run_simm, verbose):
+        if not self.initialized:
+            self.prepare_symmetrization()
+        if len(self.a_coeffs) > 0:
+            raise NotImplementedError("the device recurrence starts a fresh chain; call reset()")
+        psi_norm = np.sum(self.psi ** 2)
+        if np.isnan(psi_norm) or psi_norm == 0:
+            raise ValueError("Error, you must initialize a perturbation to start the Lanczos.")
+        if run_simm is None:
+            run_simm = self.use_wigner
+        if run_simm and not self.use_wigner:
+            raise NotImplementedError("The symmetric Lanczos works only with Wigner.")
+        psi = np.ascontiguousarray(self.psi[None, :], dtype=TYPE_DP)
+        a = np.zeros((1, n_iter))
+        b = np.zeros((1, n_iter))
+        c = np.zeros((1, n_iter))
+        nd = np.zeros(1, dtype=np.int32)
+        ms = C.c_double(0.0)
+        flags = (int(self.ignore_harmonic), int(self.ignore_v3), int(self.ignore_v4))
+        prec = precision_code(self.precision)
+        if run_simm:
+            if self.reverse_L or self.shift_value:
+                raise NotImplementedError("the device symmetric recurrence has no reverse/shift")
+            rc = _lib.lib().tdscha_lanczos_run(
+                self.engine.ctx, 1, _p(psi), int(n_iter), None, EPSILON, 0, prec, *flags, 1,
+                _p(a), _p(b), _p(nd), None, C.byref(ms))
+            _lib.check(rc, "tdscha_lanczos_run")
+            k = int(nd[0])
+            nb = k if (k == n_iter or abs(b[0, k - 1]) >= EPSILON) else k - 1
+            self.a_coeffs, self.b_coeffs = list(a[0, :k]), list(b[0, :nb])
+            self.c_coeffs = list(b[0, :nb])
+        else:
+            rc = _lib.lib().tdscha_lanczos_run_biconjugate(
+                self.engine.ctx, 1, _p(psi), int(n_iter), None, EPSILON, int(self.use_wigner),
+                prec, *flags, int(bool(self.reverse_L)), float(self.shift_value), 1, _p(a),
+                _p(b), _p(c), _p(nd), C.byref(ms))
+            _lib.check(rc, "tdscha_lanczos_run_biconjugate")
+            k = int(nd[0])
+            # a chain that stopped on a small b or c leaves that slot zero
+            kept = abs(b[0, k - 1]) >= EPSILON and abs(c[0, k - 1]) >= EPSILON
+            nb = k if kept else k - 1
+            self.a_coeffs, self.b_coeffs = list(a[0, :k]), list(b[0, :nb])
+            self.c_coeffs = list(c[0, :nb])
+        if verbose:
+            for i in range(len(self.a_coeffs)):
+                print("a_%d = %.8e" % (i, self.a_coeffs[i]))
+        return len(self.b_coeffs) < len(self.a_coeffs)
 
     def run_FT_batched(self, psis, n_iter, caps=None, tol=EPSILON, reorth=True,
                        precision=None, use_graph=True, return_basis=False):
