@@ -295,6 +295,14 @@ def main():
         lanc.engine.synchronize()
         return chains, time.perf_counter() - t0, dev_ms
 
+    # K splits of the outer-product GEMMs chosen by the autotuner for this
+    # batch (untimed); both split precisions share the split path's choice
+    tuned = {}
+    for prec in dict.fromkeys([args.precision] + ([] if args.no_fp64 else ["fp64"])):
+        ks, ms = lanc.engine.autotune(len(psis), prec, reps=2)
+        tuned[prec] = {"ksplit": ks, "ms_per_pass": ms}
+        log("autotune %s: K split %d, %.3f ms per batched pass" % (prec, ks, ms))
+
     results = {}
     precs = [args.precision] + ([] if (args.no_fp64 or args.precision == "fp64") else ["fp64"])
     if args.precision == "fp16x2" and not args.no_fp64:
@@ -471,6 +479,7 @@ def main():
         "accuracy": acc.get(args.precision),
         "bf16x2": bf16,
         "kernel_ms": {p: [float(x) for x in prof[p]] for p in prof},
+        "autotuned_ksplit": tuned,
         "roofline": roof,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,

@@ -59,6 +59,17 @@ int tdscha_ctx_info(const tdscha_ctx* ctx, int* n_modes, int* n_modes_padded, lo
  */
 int tdscha_set_ksplit(tdscha_ctx* ctx, int ksplit);
 
+/*
+ * Autotuner of the real-space perturbed averages (no counterpart in the
+ * reference): times one pass of n_chains (1..8) perturbations at the given
+ * precision for each K split around the default (multiples of 8; reps
+ * passes each), keeps the fastest for that precision and returns it in
+ * ksplit_out with its time in best_ms.  Every K split computes the same sums
+ * in another order.
+ */
+int tdscha_autotune(tdscha_ctx* ctx, int n_chains, int precision, int reps, int* ksplit_out,
+                    double* best_ms);
+
 /* Wait for all work queued on the context's stream. */
 int tdscha_synchronize(tdscha_ctx* ctx);
 
@@ -180,11 +191,14 @@ int tdscha_apply_L1_inverse(tdscha_ctx* ctx, int n_vec, const double* psi, doubl
 /*
  * Device-resident symmetric Lanczos in the Wigner representation
  * (run_FT with run_simm, Modules/DynamicalLanczos.py:6032-6240) for n_chains
- * (1..8) starting vectors psi0 [n_chains, n + n(n+1)] (host), batched through
- * every L application.  Each chain stops at |b| < tol or after caps[c] steps
- * (caps may be NULL: n_steps for all).  reorth = 1 adds full classical
- * Gram-Schmidt reorthogonalisation (two passes) against the whole basis.
- * use_graph = 1 captures one step as a hipGraph and replays it.
+ * (1..64) starting vectors psi0 [n_chains, n + n(n+1)] (host), batched through
+ * every L application (8 chains per perturbed-average pass).  Each chain
+ * stops at |b| < tol or after caps[c] steps (caps may be NULL: n_steps for
+ * all).  reorth = 1 adds full classical Gram-Schmidt reorthogonalisation
+ * (two passes) against the whole basis.  use_graph = 1 captures one step as
+ * a hipGraph and replays it; 2 captures the whole run and launches it once.
+ * The same limits and use_graph values hold for
+ * tdscha_lanczos_run_biconjugate below.
  * Outputs: a_out, b_out [n_chains, n_steps]; n_done [n_chains] = number of a
  * coefficients; basis_out (optional) [n_chains, n_steps + 1, len];
  * elapsed_ms (optional) device time of the step loop.

@@ -59,6 +59,7 @@ def _signatures(L):
                                             C.c_int, C.c_double, C.c_double, _vp]
     L.tdscha_profile_perturb_averages.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _vp]
     L.tdscha_set_ksplit.argtypes = [_vp, C.c_int]
+    L.tdscha_autotune.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _ip, C.POINTER(C.c_double)]
     L.tdscha_device_alloc.argtypes = [C.POINTER(_vp), C.c_size_t]
     L.tdscha_device_free.argtypes = [_vp]
     L.tdscha_memcpy.argtypes = [_vp, _vp, C.c_size_t, C.c_int]

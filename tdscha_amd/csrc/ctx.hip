@@ -18,6 +18,8 @@ __global__ void expand_images(const double*, const double*, const double*, const
                               int, int, long, double*, double*, double*);
 int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* ev);
 int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s, hipEvent_t* ev);
+int set_ksplit_split(Ctx* c, int ks);
+int default_ksplit_split(Ctx* c);
 
 // pad R1 (n) -> (np) and Y1 (n,n) -> symmetric (np,np); zero outside
 __global__ void pad_inputs(const double* __restrict__ R1, const double* __restrict__ Y1, int n,
@@ -435,6 +437,56 @@ int tdscha_set_ksplit(tdscha_ctx* ctx, int ksplit) {
   c->zpart = zp;
   c->fpart = fp;
   c->ksplit = ksplit;
+  return 0;
+}
+
+int tdscha_autotune(tdscha_ctx* ctx, int n_chains, int precision, int reps, int* ksplit_out,
+                    double* best_ms) {
+  TDS_REQUIRE(ctx != nullptr, 2, "tdscha_autotune: NULL context");
+  TDS_REQUIRE(n_chains >= 1 && n_chains <= kMaxChains && reps >= 1, 2,
+              "tdscha_autotune: n_chains must be in [1, 8] and reps >= 1");
+  TDS_REQUIRE(precision >= 0 && precision <= 2, 2, "precision must be 0, 1 or 2");
+  Ctx* c = reinterpret_cast<Ctx*>(ctx);
+  if (precision != PREC_FP64) {
+    const int rc = ensure_split(c, precision);
+    if (rc) return rc;
+  }
+  // candidates: the default and its neighbours (multiples of 8), each timed
+  // on reps passes of this batch; the fastest total is kept
+  const int ks0 = precision == PREC_FP64 ? c->ksplit : default_ksplit_split(c);
+  std::vector<int> cand;
+  for (int f : {1, 2, 3, 4, 6}) {
+    for (int ks : {ks0 * f / 4, ks0 * f / 2}) {
+      ks = std::max(8, ks / 8 * 8);
+      if (std::find(cand.begin(), cand.end(), ks) == cand.end()) cand.push_back(ks);
+    }
+  }
+  std::sort(cand.begin(), cand.end());
+  double best = 1e300;
+  int best_ks = ks0;
+  for (int ks : cand) {
+    int rc;
+    if (precision == PREC_FP64) {
+      if ((long)ks * 256 > c->Mp) continue;
+      rc = tdscha_set_ksplit(ctx, ks);
+    } else {
+      if ((long)ks * 4 > c->Mp / 16) continue;
+      rc = set_ksplit_split(c, ks);
+    }
+    if (rc) return rc;
+    double ms[4];
+    rc = tdscha_profile_perturb_averages(ctx, n_chains, precision, reps, ms);
+    if (rc) return rc;
+    if (ms[3] < best) {
+      best = ms[3];
+      best_ks = ks;
+    }
+  }
+  const int rc = precision == PREC_FP64 ? tdscha_set_ksplit(ctx, best_ks)
+                                        : set_ksplit_split(c, best_ks);
+  if (rc) return rc;
+  if (ksplit_out) *ksplit_out = best_ks;
+  if (best_ms) *best_ms = best;
   return 0;
 }
 

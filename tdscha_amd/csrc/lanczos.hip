@@ -2,13 +2,14 @@
 //
 // The Wigner-representation recurrence of run_FT with run_simm
 // (Modules/DynamicalLanczos.py:6032-6240, p = q, metric = mask_dot_wigner,
-// DL:5905-5954), batched over up to 8 chains that share every L application,
+// DL:5905-5954), batched over up to 64 chains (L applied 8 at a time),
 // optionally with full classical Gram-Schmidt reorthogonalisation repeated
 // twice (the real-space analogue of the q-space full reorthogonalisation,
 // Modules/QSpaceLanczos.py:1113-1130).  Chains stop independently at
 // |b| < tol or at their own step cap; every scalar lives on the device so
-// one step is a fixed kernel sequence, captured once as a hipGraph and
-// replayed.  Reductions are two-stage with a fixed order (deterministic).
+// one step is a fixed kernel sequence: captured once as a hipGraph and
+// replayed, or the whole run captured and launched once.  Reductions are
+// two-stage with a fixed order (deterministic).
 #include <cmath>
 #include <vector>
 
@@ -24,6 +25,20 @@ int setup_pairs(Ctx* c);
 
 constexpr int kRedBlocks = 128;   // blocks per chain in the partial reductions
 constexpr int kRedThreads = 256;
+constexpr int kMaxRunChains = 64; // chains of one device Lanczos run
+
+// L applied to nc chains of the run, kMaxChains at a time (the perturbed
+// averages take up to kMaxChains per pass)
+static void apply_L_groups(Ctx* c, int nc, long len, const double* in, double* out,
+                           int use_wigner, int transpose, int ignore_harmonic, int ignore_v3,
+                           int ignore_v4, int reverse, double shift, int precision,
+                           hipStream_t s) {
+  for (int c0 = 0; c0 < nc; c0 += kMaxChains) {
+    const int g = nc - c0 < kMaxChains ? nc - c0 : kMaxChains;
+    apply_L_dev(c, g, in + (long)c0 * len, out + (long)c0 * len, use_wigner, transpose,
+                ignore_harmonic, ignore_v3, ignore_v4, reverse, shift, precision, s);
+  }
+}
 
 struct LanczosState {
   int nc = 0, kmax = 0;
@@ -299,8 +314,8 @@ static void lanczos_step(Ctx* c, LanczosState& st, int reorth, int precision, in
   const long ps = (long)(kmax + 1) * kRedBlocks;
   const dim3 gv((unsigned)((len + 255) / 256), nc);
   const dim3 gr(kRedBlocks, nc);
-  apply_L_dev(c, nc, st.qcur, st.Lq, 1, 0, ignore_harmonic, ignore_v3, ignore_v4, 0, 0.0,
-              precision, s);
+  apply_L_groups(c, nc, len, st.qcur, st.Lq, 1, 0, ignore_harmonic, ignore_v3, ignore_v4, 0,
+                 0.0, precision, s);
   hipLaunchKernelGGL(dot_partial, gr, dim3(kRedThreads), 0, s, st.qcur, len, st.Lq, len, st.mask,
                      len, st.part, ps, 0);
   hipLaunchKernelGGL(lanczos_residual, gv, dim3(256), 0, s, st.part, ps, st.Q, st.qcur, st.Lq,
@@ -523,10 +538,10 @@ static void bi_step(Ctx* c, BiState& st, int use_wigner, int precision, int igno
   const long ps = 4L * kRedBlocks;
   const dim3 gv((unsigned)((len + 255) / 256), nc);
   const dim3 gr(kRedBlocks, nc);
-  apply_L_dev(c, nc, st.q, st.Lq, use_wigner, 0, ignore_harmonic, ignore_v3, ignore_v4, reverse,
-              shift, precision, s);
-  apply_L_dev(c, nc, st.p, st.pL, use_wigner, 1, ignore_harmonic, ignore_v3, ignore_v4, reverse,
-              shift, precision, s);
+  apply_L_groups(c, nc, len, st.q, st.Lq, use_wigner, 0, ignore_harmonic, ignore_v3, ignore_v4,
+                 reverse, shift, precision, s);
+  apply_L_groups(c, nc, len, st.p, st.pL, use_wigner, 1, ignore_harmonic, ignore_v3, ignore_v4,
+                 reverse, shift, precision, s);
   hipLaunchKernelGGL(dot_partial, gr, dim3(kRedThreads), 0, s, st.p, len, st.Lq, len, st.ones,
                      len, st.part, ps, 0);
   hipLaunchKernelGGL(bi_residual, gv, dim3(256), 0, s, st.part, ps, st.q, st.qprev, st.p,
@@ -553,7 +568,8 @@ int tdscha_lanczos_run_biconjugate(tdscha_ctx* ctx, int n_chains, const double* 
                                    double* a_out, double* b_out, double* c_out, int* n_done,
                                    double* elapsed_ms) {
   TDS_REQUIRE(ctx != nullptr, 2, "tdscha_lanczos_run_biconjugate: NULL context");
-  TDS_REQUIRE(n_chains >= 1 && n_chains <= kMaxChains, 2, "n_chains must be in [1, 8]");
+  TDS_REQUIRE(n_chains >= 1 && n_chains <= kMaxRunChains, 2, "n_chains must be in [1, 64]");
+  TDS_REQUIRE(use_graph >= 0 && use_graph <= 2, 2, "use_graph must be 0, 1 or 2");
   TDS_REQUIRE(n_steps >= 1 && n_steps <= 100000, 2, "n_steps must be in [1, 100000]");
   TDS_REQUIRE(psi0 && a_out && b_out && c_out && n_done, 2,
               "tdscha_lanczos_run_biconjugate: NULL pointer");
@@ -628,17 +644,19 @@ int tdscha_lanczos_run_biconjugate(tdscha_ctx* ctx, int n_chains, const double* 
     return fail("initialisation failed");
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return fail("event creation failed");
+  const int captured = use_graph == 2 ? n_steps : 1;
   if (use_graph) {
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return fail("stream capture failed to start");
-    bi_step(c, st, use_wigner, precision, ignore_harmonic, ignore_v3, ignore_v4, reverse, shift,
-            tol, s);
+    for (int step = 0; step < captured; ++step)
+      bi_step(c, st, use_wigner, precision, ignore_harmonic, ignore_v3, ignore_v4, reverse, shift,
+              tol, s);
     if (hipStreamEndCapture(s, &graph) != hipSuccess) return fail("stream capture failed");
     if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
       return fail("graph instantiation failed");
   }
   (void)hipEventRecord(e0, s);
-  for (int step = 0; step < n_steps; ++step) {
+  for (int step = 0; step < n_steps; step += use_graph ? captured : 1) {
     if (use_graph) {
       if (hipGraphLaunch(exec, s) != hipSuccess) return fail("graph launch failed");
     } else {
@@ -671,7 +689,8 @@ int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_
                        double* a_out, double* b_out, int* n_done, double* basis_out,
                        double* elapsed_ms) {
   TDS_REQUIRE(ctx != nullptr, 2, "tdscha_lanczos_run: NULL context");
-  TDS_REQUIRE(n_chains >= 1 && n_chains <= kMaxChains, 2, "n_chains must be in [1, 8]");
+  TDS_REQUIRE(n_chains >= 1 && n_chains <= kMaxRunChains, 2, "n_chains must be in [1, 64]");
+  TDS_REQUIRE(use_graph >= 0 && use_graph <= 2, 2, "use_graph must be 0, 1 or 2");
   TDS_REQUIRE(n_steps >= 1 && n_steps <= 100000, 2, "n_steps must be in [1, 100000]");
   TDS_REQUIRE(psi0 && a_out && b_out && n_done, 2, "tdscha_lanczos_run: NULL pointer");
   TDS_REQUIRE(precision >= 0 && precision <= 2, 2, "precision must be 0, 1 or 2");
@@ -735,8 +754,9 @@ int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_
     set_error(std::string("tdscha_lanczos_run: ") + what);
     return -1;
   };
+  // psi0 staged in the residual buffer (n_chains vectors)
   if (hipMemcpyAsync(st.cap, hcap.data(), sizeof(int) * n_chains, hipMemcpyHostToDevice, s) != hipSuccess ||
-      hipMemcpyAsync(c->psi_dev, psi0, sizeof(double) * n_chains * len, hipMemcpyHostToDevice, s) != hipSuccess)
+      hipMemcpyAsync(st.r, psi0, sizeof(double) * n_chains * len, hipMemcpyHostToDevice, s) != hipSuccess)
     return fail("upload failed");
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, st.k, n_chains, 0);
   hipLaunchKernelGGL(fill_int, dim3(1), dim3(64), 0, s, st.active, n_chains, 1);
@@ -751,24 +771,29 @@ int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_
   hipLaunchKernelGGL(build_mask, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, s, n, len,
                      st.mask);
   const long ps = (long)(n_steps + 1) * kRedBlocks;
-  hipLaunchKernelGGL(dot_partial, dim3(kRedBlocks, n_chains), dim3(kRedThreads), 0, s, c->psi_dev,
-                     len, c->psi_dev, len, st.mask, len, st.part, ps, 0);
+  hipLaunchKernelGGL(dot_partial, dim3(kRedBlocks, n_chains), dim3(kRedThreads), 0, s, st.r,
+                     len, st.r, len, st.mask, len, st.part, ps, 0);
   hipLaunchKernelGGL(lanczos_init, dim3((unsigned)((len + 255) / 256), n_chains), dim3(256), 0, s,
-                     st.part, ps, c->psi_dev, len, n_steps, st.Q, st.qcur);
+                     st.part, ps, st.r, len, n_steps, st.Q, st.qcur);
   if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
     return fail("initialisation failed");
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     return fail("event creation failed");
+  // use_graph 1: one step captured and replayed; 2: the whole run captured
+  // and launched once (stopped chains fall through every later step on
+  // their device flags)
+  const int captured = use_graph == 2 ? n_steps : 1;
   if (use_graph) {
     if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return fail("stream capture failed to start");
-    lanczos_step(c, st, reorth, precision, ignore_harmonic, ignore_v3, ignore_v4, tol, s);
+    for (int step = 0; step < captured; ++step)
+      lanczos_step(c, st, reorth, precision, ignore_harmonic, ignore_v3, ignore_v4, tol, s);
     if (hipStreamEndCapture(s, &graph) != hipSuccess) return fail("stream capture failed");
     if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
       return fail("graph instantiation failed");
   }
   (void)hipEventRecord(e0, s);
-  for (int step = 0; step < n_steps; ++step) {
+  for (int step = 0; step < n_steps; step += use_graph ? captured : 1) {
     if (use_graph) {
       if (hipGraphLaunch(exec, s) != hipSuccess) return fail("graph launch failed");
     } else {

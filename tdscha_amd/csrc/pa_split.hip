@@ -1034,6 +1034,31 @@ static int launch_all(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* e
   return 0;
 }
 
+// K split of the split-precision outer GEMM (work-group K ranges are whole
+// units of 4 steps); re-allocates the partial tiles
+int set_ksplit_split(Ctx* c, int ks) {
+  TDS_REQUIRE(c->coef32 != nullptr, 2, "set_ksplit_split: split path not prepared");
+  TDS_REQUIRE(ks >= 8 && ks % 8 == 0 && (long)ks * 4 <= c->Mp / kBK, 2,
+              "set_ksplit_split: K split must be a multiple of 8 with at least 4 steps per range");
+  TDS_CHECK_HIP(hipStreamSynchronize(c->stream));
+  float* zp = nullptr;
+  double* fp = nullptr;
+  TDS_CHECK_HIP(hipMalloc((void**)&zp, sizeof(float) * kMaxChains * (size_t)ks * c->np * c->np));
+  if (hipMalloc((void**)&fp, sizeof(double) * kMaxChains * (size_t)ks * c->np) != hipSuccess) {
+    (void)hipFree(zp);
+    set_error("set_ksplit_split: hipMalloc failed");
+    return -1;
+  }
+  (void)hipFree(c->zpart32);
+  (void)hipFree(c->fpart_s);
+  c->zpart32 = zp;
+  c->fpart_s = fp;
+  c->ksplit_s = ks;
+  return 0;
+}
+
+int default_ksplit_split(Ctx* c) { return choose_ksplit_split(c->np / kBR, c->Mp); }
+
 int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s, hipEvent_t* ev) {
   int rc = ensure_split(c, prec);
   if (rc) return rc;

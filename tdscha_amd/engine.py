@@ -158,6 +158,18 @@ class Engine:
                 b.free()
         return (f[0], d2v[0]) if single else (f, d2v)
 
+    def autotune(self, n_chains=3, precision=FP64, reps=3):
+        """Time the K splits of the outer-product GEMM for this batch and keep
+        the fastest; returns (ksplit, ms per pass)."""
+        ks = C.c_int(0)
+        ms = C.c_double(0.0)
+        _lib.check(self._L.tdscha_autotune(self.ctx, int(n_chains), precision_code(precision),
+                                           int(reps), C.byref(ks), C.byref(ms)),
+                   "tdscha_autotune")
+        if precision_code(precision) == FP64:
+            self.ksplit = ks.value
+        return ks.value, ms.value
+
     def set_ksplit(self, ksplit):
         """Number of image ranges of the FP64 outer-product GEMM (multiple of 8)."""
         _lib.check(self._L.tdscha_set_ksplit(self.ctx, int(ksplit)), "tdscha_set_ksplit")

@@ -512,11 +512,14 @@ class Lanczos(object):
 
     def run_FT_batched(self, psis, n_iter, caps=None, tol=EPSILON, reorth=True,
                        precision=None, use_graph=True, return_basis=False):
-        """Device-resident symmetric (Wigner) Lanczos for up to 8 chains at once.
+        """Device-resident symmetric (Wigner) Lanczos for up to 64 chains at once.
 
         Returns a list of (a, b) coefficient arrays per chain and the device
-        time in ms.  Each chain stops at |b| < tol or at its cap.
+        time in ms.  Each chain stops at |b| < tol or at its cap.  use_graph:
+        True or "step" replays a captured step, "run" captures the whole run
+        as one graph, False launches kernels plainly.
         """
+        graph_mode = {False: 0, True: 1, "step": 1, "run": 2}[use_graph]
         if not self.use_wigner:
             raise NotImplementedError("the batched device Lanczos is the symmetric Wigner one")
         psis = np.ascontiguousarray(np.atleast_2d(psis), dtype=TYPE_DP)
@@ -533,7 +536,7 @@ class Lanczos(object):
         rc = _lib.lib().tdscha_lanczos_run(
             self.engine.ctx, nc, _p(psis), int(n_iter), None if capv is None else _p(capv),
             float(tol), int(bool(reorth)), prec, int(self.ignore_harmonic), int(self.ignore_v3),
-            int(self.ignore_v4), int(bool(use_graph)), _p(a), _p(b), _p(nd),
+            int(self.ignore_v4), graph_mode, _p(a), _p(b), _p(nd),
             None if basis is None else _p(basis), C.byref(ms))
         _lib.check(rc, "tdscha_lanczos_run")
         chains = []
