@@ -218,10 +218,13 @@ def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True, tiles
             worst50 = max(worst50, float(np.max(d[:50])))
         wall1 = time.perf_counter() - t
         # The per-chain run sums in the order of its own tiles: the first 50
-        # coefficients agree to rounding; late coefficients of a few chains
-        # depend on that order at the 1e-2 level past step ~200 (rounding
-        # differences grow along long recurrences).  With equal tiles the runs are
-        # bit-identical (tests/test_gpu_qspace.py).
+        # coefficients agree to rounding.  On this grid the chains start from
+        # complex Bloch vectors, where the q-space operator (QL:398-400,
+        # JQ:509-517) is neither complex-linear nor Hermitian in its metric, so
+        # the Hermitian recurrence is not a projection of it and nothing damps
+        # rounding differences: late coefficients of a few chains depend on the
+        # summation order at the 1e-2 level past step ~200.  With equal tiles
+        # the runs are bit-identical (tests/test_gpu_qspace.py).
         out.update({"per_chain_value": n_l / wall1, "per_chain_wall_s": wall1,
                     "per_chain_tiles": list(tiles1), "per_chain_d2v_form": eng.d2v_form(),
                     "per_chain_vs_batched_max_rel_diff_a_first50": worst50,

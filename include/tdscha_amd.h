@@ -216,8 +216,9 @@ int tdscha_lanczos_run(tdscha_ctx* ctx, int n_chains, const double* psi0, int n_
  * (transpose semantics of tdscha_apply_L, in the standard or the Wigner
  * representation), the reference's a, b, c and s_norm bookkeeping.  Chains
  * stop when |b| or |c| < tol (a kept, b and c dropped) or at caps[c] steps
- * (caps may be NULL).  reverse / shift as tdscha_apply_L.  One step is
- * captured as a hipGraph and replayed with use_graph.  a_out, b_out, c_out
+ * (caps may be NULL).  reverse / shift as tdscha_apply_L.  use_graph as
+ * tdscha_lanczos_run (0 plain launches, 1 one step captured and replayed,
+ * 2 the whole run captured as one graph).  a_out, b_out, c_out
  * [n_chains, n_steps]; n_done[c] = number of a coefficients.
  */
 int tdscha_lanczos_run_biconjugate(tdscha_ctx* ctx, int n_chains, const double* psi0,
