@@ -346,18 +346,17 @@ __global__ __launch_bounds__(256, 1) void weights_split(
     }
   };
 
-  v16f acc[NC][4];
-#pragma unroll
-  for (int c = 0; c < NC; ++c)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[c][i] = v16f{};
+  v16f acc[NC][4];   // set by the first step of every tile
 
   // step g (slot K % 4) with its fragments already in `cur`; reads the
   // fragments of step g + 1 into `nxt`.  Every step issues exactly kBlocks
   // loads per wave (the copy of step g + 3), so vmcnt(kBlocks) at its start
   // means the copy of step g + 1 has landed.
-  auto step = [&](auto kk, auto prefetch, int g, Frags& cur, Frags& nxt) {
+  // The first step of a tile starts its accumulators from the zero operand of
+  // the MFMA instead of zeroed registers.
+  auto step = [&](auto kk, auto prefetch, int g, Frags& cur, Frags& nxt, auto first) {
     constexpr int K = decltype(kk)::value;
+    constexpr bool kFirst = decltype(first)::value;
     TDS_WAIT_VMCNT(kBlocks);
     lds_barrier();
     // slot (K + 3) % 4 was last read for step g - 1, during step g - 2 (or
@@ -373,7 +372,8 @@ __global__ __launch_bounds__(256, 1) void weights_split(
       for (int c = 0; c < NC; ++c)
 #pragma unroll
         for (int i = 0; i < 4; ++i)
-          acc[c][i] = Mma<H>::mma(cur.b[c][pr == 1 ? 1 : 0], cur.a[i][pr == 2 ? 1 : 0], acc[c][i]);
+          acc[c][i] = Mma<H>::mma(cur.b[c][pr == 1 ? 1 : 0], cur.a[i][pr == 2 ? 1 : 0],
+                                  (kFirst && pr == 0) ? v16f{} : acc[c][i]);
   };
 
   Frags f0, f1;
@@ -389,16 +389,22 @@ __global__ __launch_bounds__(256, 1) void weights_split(
     // epilogue)
     const int gt = ti * nkc;
     read(sw0, f0);
-    for (int kc = 0; kc < nkc - 4; kc += 4) {
-      step(ic<0>{}, std::true_type{}, gt + kc, f0, f1);
-      step(ic<1>{}, std::true_type{}, gt + kc + 1, f1, f0);
-      step(ic<2>{}, std::true_type{}, gt + kc + 2, f0, f1);
-      step(ic<3>{}, std::true_type{}, gt + kc + 3, f1, f0);
+    constexpr std::true_type yes{};
+    constexpr std::false_type no{};
+    step(ic<0>{}, yes, gt, f0, f1, yes);
+    step(ic<1>{}, yes, gt + 1, f1, f0, no);
+    step(ic<2>{}, yes, gt + 2, f0, f1, no);
+    step(ic<3>{}, yes, gt + 3, f1, f0, no);
+    for (int kc = 4; kc < nkc - 4; kc += 4) {
+      step(ic<0>{}, yes, gt + kc, f0, f1, no);
+      step(ic<1>{}, yes, gt + kc + 1, f1, f0, no);
+      step(ic<2>{}, yes, gt + kc + 2, f0, f1, no);
+      step(ic<3>{}, yes, gt + kc + 3, f1, f0, no);
     }
-    step(ic<0>{}, std::true_type{}, gt + nkc - 4, f0, f1);
-    step(ic<1>{}, std::true_type{}, gt + nkc - 3, f1, f0);
-    step(ic<2>{}, std::true_type{}, gt + nkc - 2, f0, f1);
-    step(ic<3>{}, std::false_type{}, gt + nkc - 1, f1, f0);
+    step(ic<0>{}, yes, gt + nkc - 4, f0, f1, no);
+    step(ic<1>{}, yes, gt + nkc - 3, f1, f0, no);
+    step(ic<2>{}, yes, gt + nkc - 2, f0, f1, no);
+    step(ic<3>{}, no, gt + nkc - 1, f1, f0, no);
 
     // ---- epilogue of tile (rt, ct).  Per image row and chain, with u, f the
     // unscaled images and V the accumulators (sy times the true V):
@@ -498,10 +504,6 @@ __global__ __launch_bounds__(256, 1) void weights_split(
       }
       lds_barrier();
     }
-#pragma unroll
-    for (int c = 0; c < NC; ++c)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) acc[c][i] = v16f{};
   }
   // the copies issued past the end of the stream land before the work-group
   // releases its LDS
