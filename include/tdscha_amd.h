@@ -39,15 +39,25 @@ const char* tdscha_last_error(void);
  *   block_modes  concatenated mode indices of the blocks (length n_modes)
  *   block_syms   n_blocks pointers to f64[n_syms, d_b, d_b] (row = output mode)
  *   sym_block_id [n_modes] block of each mode
- *   flags        reserved (0)
+ *   flags        0, or TDSCHA_CTX_ROTATE_OPERATORS: store the configurations
+ *                only and apply the symmetries by rotating the perturbation
+ *                (S_j^T R1, S_j^T Y1 S_j) and the results (S_j f, S_j d2v
+ *                S_j^T), as the reference does per image on the fly
+ *                (CModules/LanczosFunctions.c:2158-2170): the same flops in
+ *                n_syms passes over the configurations, n_syms times less
+ *                device memory.  Image ranges must then hold whole
+ *                configurations.
  */
+#define TDSCHA_CTX_ROTATE_OPERATORS 1
 int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double* X,
                       const double* Y, const double* w, const double* rho, double T, int n_syms,
                       int n_blocks, const int* block_dim, const int* block_modes,
                       const double* const* block_syms, const int* sym_block_id, int flags);
 void tdscha_ctx_destroy(tdscha_ctx* ctx);
 
-/* Padded sizes and the K-split of the outer-product GEMM. */
+/* Padded sizes and the K-split of the outer-product GEMM.  n_images counts
+ * configurations x symmetries; n_images_padded the rows stored on the device
+ * (configurations only, padded, with TDSCHA_CTX_ROTATE_OPERATORS). */
 int tdscha_ctx_info(const tdscha_ctx* ctx, int* n_modes, int* n_modes_padded, long* n_images,
                     long* n_images_padded, int* ksplit);
 

@@ -185,6 +185,21 @@ struct Ctx {
   double* stage_out = nullptr;
   // host-side copies of the symmetry description
   std::vector<int> block_dim, block_off, block_modes, sym_block_id;
+  // operator-rotation form (flag TDSCHA_CTX_ROTATE_OPERATORS): U, F hold the
+  // configurations themselves (n_syms = 1 for the kernels) and every pass is
+  // run on rotated perturbations S_j^T R1, S_j^T Y1 S_j, its results rotated
+  // back and summed over the rot_nsym symmetries
+  bool rotate = false;
+  int rot_nsym = 1;
+  int* rt_mb = nullptr;        // [n] block of each mode
+  int* rt_mp = nullptr;        // [n] position of each mode in its block
+  int* rt_bd = nullptr;        // [n_blocks] block sizes
+  int* rt_bmo = nullptr;       // [n_blocks] offset of each block in rt_bm
+  int* rt_bm = nullptr;        // [n] modes of the blocks, concatenated
+  long* rt_bso = nullptr;      // [n_blocks] offset of each block's matrices in rt_syms
+  double* rt_syms = nullptr;   // per block [rot_nsym, d, d] (row = output mode)
+  double* R1_base = nullptr;   // [kMaxChains, np] unrotated padded inputs of a call
+  double* Y1_base = nullptr;   // [kMaxChains, np, np]
 };
 
 int ensure_split(Ctx* c, int prec);

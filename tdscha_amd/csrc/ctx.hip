@@ -20,6 +20,9 @@ int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* ev);
 int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s, hipEvent_t* ev);
 int set_ksplit_split(Ctx* c, int ks);
 int default_ksplit_split(Ctx* c);
+int rot_setup(Ctx* c, const double* X_dev, const double* Y_dev, const double* rho_dev);
+int run_pa_rot(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2v_dev,
+               hipStream_t s);
 
 // pad R1 (n) -> (np) and Y1 (n,n) -> symmetric (np,np); zero outside
 __global__ void pad_inputs(const double* __restrict__ R1, const double* __restrict__ Y1, int n,
@@ -66,6 +69,7 @@ static int choose_ksplit(int ntile, long Mp) {
 // run the perturbed averages for nc chains whose padded inputs are in c->R1/c->Y1;
 // outputs: unpadded f/d2v at the given device pointers
 int run_pa(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2v_dev, hipStream_t s) {
+  if (c->rotate) return run_pa_rot(c, nc, apply_d4, prec, f_dev, d2v_dev, s);
   int rc;
   if (prec == PREC_FP64) {
     rc = launch_pa_fp64(c, nc, apply_d4, s, nullptr);
@@ -93,8 +97,9 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
                       const double* Y, const double* w, const double* rho, double T, int n_syms,
                       int n_blocks, const int* block_dim, const int* block_modes,
                       const double* const* block_syms, const int* sym_block_id, int flags) {
-  (void)flags;
   TDS_REQUIRE(out != nullptr, 2, "tdscha_ctx_create: out is NULL");
+  TDS_REQUIRE((flags & ~TDSCHA_CTX_ROTATE_OPERATORS) == 0, 2, "tdscha_ctx_create: unknown flags");
+  const bool rotate = (flags & TDSCHA_CTX_ROTATE_OPERATORS) != 0;
   *out = nullptr;
   TDS_REQUIRE(n_modes > 0 && n_configs > 0 && n_syms > 0 && n_blocks > 0, 2,
               "tdscha_ctx_create: sizes must be positive");
@@ -129,8 +134,11 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
   c->n = n_modes;
   c->np = round_up(n_modes, kTile);
   c->N = n_configs;
-  c->n_syms = n_syms;
-  c->M = (long)n_configs * n_syms;
+  // the rotation form runs the kernels on the configurations alone
+  c->rotate = rotate;
+  c->rot_nsym = n_syms;
+  c->n_syms = rotate ? 1 : n_syms;
+  c->M = (long)n_configs * c->n_syms;
   c->Mp = round_up_l(c->M, 128);
   c->row_lo = 0;
   c->row_hi = c->M;
@@ -230,7 +238,17 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
   ok = ok && hipMemcpy(d_X, X, sizeof(double) * (size_t)n_configs * n_modes, hipMemcpyHostToDevice) == hipSuccess;
   ok = ok && hipMemcpy(d_Y, Y, sizeof(double) * (size_t)n_configs * n_modes, hipMemcpyHostToDevice) == hipSuccess;
   ok = ok && hipMemcpy(d_rho, rho, sizeof(double) * n_configs, hipMemcpyHostToDevice) == hipSuccess;
-  if (ok) {
+  if (ok && rotate) {
+    // the configurations only; the symmetry tables stay for the rotations
+    ok = rot_setup(c, d_X, d_Y, d_rho) == 0 && hipStreamSynchronize(c->stream) == hipSuccess;
+    c->rt_mb = d_mb; c->rt_mp = d_mp; c->rt_bd = d_bd; c->rt_bmo = d_bmo; c->rt_bm = d_bm;
+    c->rt_bso = d_bso; c->rt_syms = d_syms;
+    d_mb = d_mp = d_bd = d_bmo = d_bm = nullptr;
+    d_bso = nullptr;
+    d_syms = nullptr;
+    ok = ok && hipMalloc((void**)&c->R1_base, sizeof(double) * kMaxChains * np) == hipSuccess &&
+         hipMalloc((void**)&c->Y1_base, sizeof(double) * kMaxChains * (long)np * np) == hipSuccess;
+  } else if (ok) {
     const int tpb = 128;
     dim3 g((unsigned)Mp, (np + tpb - 1) / tpb);
     hipLaunchKernelGGL(expand_images, g, dim3(tpb), 0, c->stream, d_X, d_Y, d_rho, d_mb, d_mp, d_bd,
@@ -259,7 +277,9 @@ void tdscha_ctx_destroy(tdscha_ctx* ctx) {
                   c->pair_row, c->pair_col, c->std_coef, c->inv_coef, c->yw, c->reaw, c->pols, c->inv_perms, c->rep_dof, c->jrep, c->ju, c->rep_sum,
                   c->tproj, c->cart_tmp, c->cart_tmp2, c->mode_tmp, c->psi_dev, c->psi_out_dev,
                   c->stage_in, c->stage_out, c->f_fin, c->d2v_fin, c->f_proj, c->epi,
-                  c->coef32, c->wpart, c->zpart32, c->fpart_s, c->sbsc, c->pmax};
+                  c->coef32, c->wpart, c->zpart32, c->fpart_s, c->sbsc, c->pmax,
+                  c->rt_mb, c->rt_mp, c->rt_bd, c->rt_bmo, c->rt_bm, c->rt_bso, c->rt_syms,
+                  c->R1_base, c->Y1_base};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& s : c->split) {
@@ -277,7 +297,9 @@ int tdscha_ctx_info(const tdscha_ctx* ctx, int* n_modes, int* n_modes_padded, lo
   const Ctx* c = reinterpret_cast<const Ctx*>(ctx);
   if (n_modes) *n_modes = c->n;
   if (n_modes_padded) *n_modes_padded = c->np;
-  if (n_images) *n_images = c->M;
+  // images of the symmetric ensemble (in the rotation form: configurations
+  // times symmetries, none of them stored) and the stored rows
+  if (n_images) *n_images = c->M * (c->rotate ? c->rot_nsym : 1);
   if (n_images_padded) *n_images_padded = c->Mp;
   if (ksplit) *ksplit = c->ksplit;
   return 0;
@@ -311,11 +333,16 @@ int tdscha_perturb_averages_range(tdscha_ctx* ctx, int n_chains, const double* R
   TDS_REQUIRE(start_index <= end_index, 2,
               "tdscha_perturb_averages_range: start_index must not exceed end_index");
   TDS_REQUIRE(start_index >= 1, 2, "tdscha_perturb_averages_range: start_index must be >= 1");
-  TDS_REQUIRE(end_index <= c->M, 2,
+  const long ns = c->rotate ? c->rot_nsym : 1;
+  TDS_REQUIRE(end_index <= c->M * ns, 2,
               "tdscha_perturb_averages_range: end_index cannot exceed n_configs * n_syms");
+  // the rotation form sums all symmetries of a configuration in one pass
+  TDS_REQUIRE(!c->rotate || ((start_index - 1) % ns == 0 && end_index % ns == 0), 2,
+              "tdscha_perturb_averages_range: with rotated operators the range must hold whole "
+              "configurations (start_index - 1 and end_index multiples of n_syms)");
   // kernel arguments are taken at launch, so the range applies to this call only
-  c->row_lo = start_index - 1;
-  c->row_hi = end_index;
+  c->row_lo = (start_index - 1) / ns;
+  c->row_hi = end_index / ns;
   const int rc = tdscha_perturb_averages_dev(ctx, n_chains, R1_dev, Y1_dev, apply_d4, precision,
                                              f_dev, d2v_dev, stream);
   c->row_lo = 0;

@@ -28,10 +28,20 @@ def _ptr(a):
     return C.c_void_p(a.ctypes.data)
 
 
-class Engine:
-    """Ensemble (X, Y, rho), frequencies, temperature and block symmetries."""
+ROTATE_OPERATORS = 1     # TDSCHA_CTX_ROTATE_OPERATORS
 
-    def __init__(self, X, Y, w, rho, T, symmetries, degenerate_space, sym_block_id):
+
+class Engine:
+    """Ensemble (X, Y, rho), frequencies, temperature and block symmetries.
+
+    ``rotate=False`` materialises the symmetric images S_j x_i, S_j y_i on the
+    device once (fastest while they fit); ``rotate=True`` keeps the
+    configurations only and rotates the perturbation and the results instead
+    (the reference's per-image application of S_j,
+    CModules/LanczosFunctions.c:2158-2170): the same results and flops with
+    n_syms times less device memory."""
+
+    def __init__(self, X, Y, w, rho, T, symmetries, degenerate_space, sym_block_id, rotate=False):
         L = _lib.lib()
         self._L = L
         X = np.ascontiguousarray(X, np.float64)
@@ -60,9 +70,11 @@ class Engine:
         ctx = C.c_void_p()
         rc = L.tdscha_ctx_create(C.byref(ctx), n, N, _ptr(X), _ptr(Y), _ptr(w), _ptr(rho),
                                  float(T), n_syms, len(blocks), _ptr(dims), _ptr(modes),
-                                 C.cast(ptrs, C.c_void_p), _ptr(sbi), 0)
+                                 C.cast(ptrs, C.c_void_p), _ptr(sbi),
+                                 ROTATE_OPERATORS if rotate else 0)
         _lib.check(rc, "tdscha_ctx_create")
         self.ctx = ctx
+        self.rotate = bool(rotate)
         self.n_modes = n
         self.n_configs = N
         self.n_syms = n_syms

@@ -35,6 +35,9 @@ from .engine import Engine, precision_code
 TYPE_DP = np.double
 EPSILON = 1e-12
 RY_TO_K = 157887.32400374097
+# largest image ensemble (U and F in FP64) that symmetry_form "auto" keeps on
+# the device; beyond it the symmetries are applied by rotating operators
+IMAGE_BUDGET_BYTES = 64 * 2 ** 30
 
 MODE_FAST_SERIAL = 1   # names kept for API compatibility; all run on the GPU
 MODE_FAST_MPI = 2
@@ -114,6 +117,11 @@ class Lanczos(object):
         self.use_wigner = use_wigner
         self.ignore_small_w = False
         self.precision = "fp64"
+        # "images": the symmetric images live on the device; "operators": the
+        # configurations only, symmetries applied by rotating the perturbation
+        # and the results; "auto": images while they take at most
+        # IMAGE_BUDGET_BYTES
+        self.symmetry_form = "auto"
         self.ensemble = None
         self.engine = None
         self._sym_source = None
@@ -229,11 +237,24 @@ class Lanczos(object):
                 self.trans_cart_perms = None
         self._upload()
 
+    def rotates_operators(self):
+        """Whether the engine applies the symmetries by rotating operators
+        (see symmetry_form)."""
+        form = self.symmetry_form
+        if form not in ("auto", "images", "operators"):
+            raise ValueError("symmetry_form must be 'auto', 'images' or 'operators'")
+        if form == "auto":
+            np_ = -(-self.n_modes // 128) * 128
+            image_bytes = 16.0 * self.N * self.n_syms * np_     # U and F in FP64
+            return self.n_syms > 1 and image_bytes > IMAGE_BUDGET_BYTES
+        return form == "operators" and self.n_syms > 1
+
     def _upload(self):
         if self.engine is not None:
             self.engine.close()
         self.engine = Engine(self.X, self.Y, self.w, self.rho, self.T, self.symmetries,
-                             self.degenerate_space, self.sym_block_id)
+                             self.degenerate_space, self.sym_block_id,
+                             rotate=self.rotates_operators())
         L = _lib.lib()
         if self.trans_projector is not None:
             perms = np.ascontiguousarray(np.array(self.trans_cart_perms), dtype=np.int32)
