@@ -234,6 +234,55 @@ def qspace_run(ens, seed=3, tol=1e-8, cap_range=(50, 400), per_chain=True, tiles
     return out
 
 
+def rotation_run(ens, psis, lanc_img, results, args):
+    """The same chains with the symmetries applied by rotating operators
+    (configurations only on the device): L-applications/s of the FP64 and
+    headline split paths, device bytes of the two ensembles, and how far the
+    rotated L and the first coefficients sit from the image form's."""
+    from tdscha_amd.lanczos import Lanczos
+    lanc = Lanczos(ens)
+    lanc.gamma_only = True
+    lanc.symmetry_form = "operators"
+    t = time.time()
+    lanc.init(use_symmetries=True)
+    log("rotation form: init %.1fs, rows stored %d (images form %d)" % (
+        time.time() - t, lanc.engine.n_images_padded, lanc_img.engine.n_images_padded))
+    nc = len(psis)
+    out = {"stored_rows": lanc.engine.n_images_padded,
+           "images_form_rows": lanc_img.engine.n_images_padded,
+           "ensemble_bytes": 16 * lanc.engine.n_images_padded * lanc.engine.n_modes_padded,
+           "images_form_bytes": 16 * lanc_img.engine.n_images_padded
+           * lanc_img.engine.n_modes_padded}
+    # a pass carries up to 8 (chain, symmetry) pairs
+    for prec in dict.fromkeys(["fp64", args.precision]):
+        lanc.engine.autotune(8, prec, reps=2)
+        lanc.run_FT_batched(psis, 1, reorth=True, precision=prec)
+        lanc.engine.synchronize()
+        t0 = time.perf_counter()
+        chains, _ = lanc.run_FT_batched(psis, args.rotation_steps, reorth=True, precision=prec)
+        lanc.engine.synchronize()
+        wall = time.perf_counter() - t0
+        n_l = sum(len(a) for a, _ in chains)
+        out[prec + "_value"] = n_l / wall
+        if prec in results:
+            img = results[prec]
+            out[prec + "_vs_images"] = (n_l / wall) / (img["n_l"] / img["wall"])
+            da = 0.0
+            for (a1, _), (a0, _) in zip(chains, img["chains"]):
+                k = min(len(a1), len(a0))
+                da = max(da, float(np.max(np.abs(a1[:k] - a0[:k]) / np.abs(a0[:k]))))
+            out[prec + "_coeff_a_max_rel_diff_vs_images"] = da
+        log("rotation form %s: %d steps x %d chains %.3fs -> %.1f L/s" % (
+            prec, args.rotation_steps, nc, wall, n_l / wall))
+    lanc.precision = "fp64"
+    lanc_img.precision = "fp64"
+    Lr, Li = lanc.apply_L_batch(psis), lanc_img.apply_L_batch(psis)
+    out["fp64_Lv_rel_diff_vs_images"] = float(np.max(np.linalg.norm(Lr - Li, axis=1)
+                                                     / np.linalg.norm(Li, axis=1)))
+    lanc.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +308,9 @@ def main():
                     help="skip the per-chain q-space comparison run")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="bytes of one device-to-device copy to run first (counter scaling)")
+    ap.add_argument("--no-rotation", action="store_true",
+                    help="skip the operator-rotation form's measurement")
+    ap.add_argument("--rotation-steps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (the ensemble sum "
@@ -439,6 +491,10 @@ def main():
             "frac": achieved / PEAK[args.precision],
             "traffic": profiled_traffic(kname, args.precision)}
 
+    rot = None
+    if not args.no_rotation:
+        rot = rotation_run(ens, psis, lanc, results, args)
+
     qres = None
     if not args.no_qspace:
         try:
@@ -486,6 +542,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "cpu_baseline_all_cores": cpu_all,
+        "operator_rotation": rot,
         "qspace": qres,
     }
     print(json.dumps(out), flush=True)

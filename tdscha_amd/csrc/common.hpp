@@ -191,15 +191,15 @@ struct Ctx {
   // back and summed over the rot_nsym symmetries
   bool rotate = false;
   int rot_nsym = 1;
-  int* rt_mb = nullptr;        // [n] block of each mode
-  int* rt_mp = nullptr;        // [n] position of each mode in its block
-  int* rt_bd = nullptr;        // [n_blocks] block sizes
-  int* rt_bmo = nullptr;       // [n_blocks] offset of each block in rt_bm
-  int* rt_bm = nullptr;        // [n] modes of the blocks, concatenated
-  long* rt_bso = nullptr;      // [n_blocks] offset of each block's matrices in rt_syms
-  double* rt_syms = nullptr;   // per block [rot_nsym, d, d] (row = output mode)
+  int rt_dmax = 0;             // largest block
+  int* rt_cols = nullptr;      // [n, dmax] modes of each mode's block
+  int* rt_d = nullptr;         // [n] size of each mode's block
+  double* rt_sr = nullptr;     // [rot_nsym, n, dmax] S_j[m, cols(m)[t]]   (row m of S_j)
+  double* rt_sc = nullptr;     // [rot_nsym, n, dmax] S_j[cols(m)[t], m]   (column m)
   double* R1_base = nullptr;   // [kMaxChains, np] unrotated padded inputs of a call
   double* Y1_base = nullptr;   // [kMaxChains, np, np]
+  double* rot_t1 = nullptr;    // [kMaxChains, np, np] rotation intermediates
+  double* rot_t2 = nullptr;
 };
 
 int ensure_split(Ctx* c, int prec);

@@ -239,15 +239,40 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
   ok = ok && hipMemcpy(d_Y, Y, sizeof(double) * (size_t)n_configs * n_modes, hipMemcpyHostToDevice) == hipSuccess;
   ok = ok && hipMemcpy(d_rho, rho, sizeof(double) * n_configs, hipMemcpyHostToDevice) == hipSuccess;
   if (ok && rotate) {
-    // the configurations only; the symmetry tables stay for the rotations
-    ok = rot_setup(c, d_X, d_Y, d_rho) == 0 && hipStreamSynchronize(c->stream) == hipSuccess;
-    c->rt_mb = d_mb; c->rt_mp = d_mp; c->rt_bd = d_bd; c->rt_bmo = d_bmo; c->rt_bm = d_bm;
-    c->rt_bso = d_bso; c->rt_syms = d_syms;
-    d_mb = d_mp = d_bd = d_bmo = d_bm = nullptr;
-    d_bso = nullptr;
-    d_syms = nullptr;
-    ok = ok && hipMalloc((void**)&c->R1_base, sizeof(double) * kMaxChains * np) == hipSuccess &&
-         hipMalloc((void**)&c->Y1_base, sizeof(double) * kMaxChains * (long)np * np) == hipSuccess;
+    // the configurations only, and per mode its block's modes and the row
+    // and column of every S_j restricted to them
+    int dmax = 0;
+    for (int b = 0; b < n_blocks; ++b) dmax = std::max(dmax, block_dim[b]);
+    std::vector<int> cols((size_t)n_modes * dmax, 0), dd(n_modes, 0);
+    std::vector<double> sr((size_t)n_syms * n_modes * dmax, 0.0), sc(sr.size(), 0.0);
+    for (int b = 0; b < n_blocks; ++b) {
+      const int d = block_dim[b];
+      const int* bm = block_modes + bmoff[b];
+      for (int a = 0; a < d; ++a) {
+        const int m = bm[a];
+        dd[m] = d;
+        for (int t = 0; t < d; ++t) cols[(size_t)m * dmax + t] = bm[t];
+        for (int j = 0; j < n_syms; ++j)
+          for (int t = 0; t < d; ++t) {
+            sr[((size_t)j * n_modes + m) * dmax + t] = block_syms[b][((size_t)j * d + a) * d + t];
+            sc[((size_t)j * n_modes + m) * dmax + t] = block_syms[b][((size_t)j * d + t) * d + a];
+          }
+      }
+    }
+    c->rt_dmax = dmax;
+    ok = hipMalloc((void**)&c->rt_cols, sizeof(int) * cols.size()) == hipSuccess &&
+         hipMalloc((void**)&c->rt_d, sizeof(int) * n_modes) == hipSuccess &&
+         hipMalloc((void**)&c->rt_sr, sizeof(double) * sr.size()) == hipSuccess &&
+         hipMalloc((void**)&c->rt_sc, sizeof(double) * sc.size()) == hipSuccess &&
+         hipMalloc((void**)&c->R1_base, sizeof(double) * kMaxChains * np) == hipSuccess &&
+         hipMalloc((void**)&c->Y1_base, sizeof(double) * kMaxChains * (long)np * np) == hipSuccess &&
+         hipMalloc((void**)&c->rot_t1, sizeof(double) * kMaxChains * (long)np * np) == hipSuccess &&
+         hipMalloc((void**)&c->rot_t2, sizeof(double) * kMaxChains * (long)np * np) == hipSuccess;
+    ok = ok && hipMemcpy(c->rt_cols, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->rt_d, dd.data(), sizeof(int) * n_modes, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->rt_sr, sr.data(), sizeof(double) * sr.size(), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->rt_sc, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && rot_setup(c, d_X, d_Y, d_rho) == 0 && hipStreamSynchronize(c->stream) == hipSuccess;
   } else if (ok) {
     const int tpb = 128;
     dim3 g((unsigned)Mp, (np + tpb - 1) / tpb);
@@ -278,8 +303,8 @@ void tdscha_ctx_destroy(tdscha_ctx* ctx) {
                   c->tproj, c->cart_tmp, c->cart_tmp2, c->mode_tmp, c->psi_dev, c->psi_out_dev,
                   c->stage_in, c->stage_out, c->f_fin, c->d2v_fin, c->f_proj, c->epi,
                   c->coef32, c->wpart, c->zpart32, c->fpart_s, c->sbsc, c->pmax,
-                  c->rt_mb, c->rt_mp, c->rt_bd, c->rt_bmo, c->rt_bm, c->rt_bso, c->rt_syms,
-                  c->R1_base, c->Y1_base};
+                  c->rt_cols, c->rt_d, c->rt_sr, c->rt_sc, c->R1_base, c->Y1_base, c->rot_t1,
+                  c->rot_t2};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& s : c->split) {

@@ -11,11 +11,17 @@
 //   s3 = (fY x_i) . (S_j^T R1)             s4 = y_i . (S_j^T R1)
 //   f   = sum_j S_j f~_j                   d2v = sum_j S_j d2v~_j S_j^T
 // where f~_j and d2v~_j are the averages over the configurations alone for the
-// rotated perturbation (S_j^T R1, S_j^T Y1 S_j).  The flops are those of the
-// image form (n_syms passes over N configurations instead of one over N n_syms
-// images); the ensemble takes n_syms times less memory.  The (chain, symmetry)
-// pairs run through the ordinary kernels 8 at a time and are summed in a fixed
-// order (chain-major, symmetry by symmetry).
+// rotated perturbation (S_j^T R1, S_j^T Y1 S_j).  The flops of the GEMMs are
+// those of the image form (n_syms passes over N configurations instead of one
+// over N n_syms images); the ensemble takes n_syms times less memory.  The
+// (chain, symmetry) pairs run through the ordinary kernels 8 at a time and
+// are summed in a fixed order (chain-major, symmetry by symmetry).
+//
+// The rotations are two one-sided block products each (S_j^T (Y1 S_j), and
+// S_j (d2v~ S_j^T)): n^2 sum_b d_b^2 / n multiply-adds per side, against
+// (sum_b d_b^2)^2 for the two-sided sum per element (blocks of up to 24 modes
+// at 128 atoms).  Per mode m the tables hold its block's modes cols(m) and
+// the row and column of S_j restricted to them.
 #include "common.hpp"
 
 namespace tdscha {
@@ -24,20 +30,13 @@ int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* ev);
 int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s, hipEvent_t* ev);
 
 struct RotTab {
-  const int* mb;      // block of each mode
-  const int* mp;      // position of each mode in its block
-  const int* bd;      // block sizes
-  const int* bmo;     // offset of each block's modes in bm
-  const int* bm;      // block modes, concatenated
-  const long* bso;    // offset of each block's matrices in syms
-  const double* syms; // per block [n_syms, d, d], row = output mode
+  const int* cols;      // [n, dmax]
+  const int* d;         // [n]
+  const double* sr;     // [nsym, n, dmax] S_j[m, cols(m)[t]]
+  const double* sc;     // [nsym, n, dmax] S_j[cols(m)[t], m]
+  int dmax;
+  int n;
 };
-
-// S_j[m, m'] for m' the k-th mode of m's block
-__device__ __forceinline__ double s_at(const RotTab& t, int j, int m, int k) {
-  const int b = t.mb[m], d = t.bd[b];
-  return t.syms[t.bso[b] + ((long)j * d + t.mp[m]) * d + k];
-}
 
 // the configurations, padded to [Mp, np] (zero rows and columns beyond)
 __global__ void rot_copy_configs(const double* __restrict__ X, const double* __restrict__ Y,
@@ -66,83 +65,116 @@ __global__ void rot_zero(long count, double* __restrict__ dst) {
     dst[e] = 0.0;
 }
 
-// pair k = blockIdx.y of the pass (p = p0 + k: chain p / nsym, symmetry
-// p % nsym): R1o = S_j^T R1, Y1o = S_j^T Y1 S_j (padded np)
-__global__ void rot_inputs(RotTab t, const double* __restrict__ R1b, const double* __restrict__ Y1b,
-                           int n, int np, int nsym, int p0, double* __restrict__ R1o,
-                           double* __restrict__ Y1o) {
-  const int k = blockIdx.y;
-  const int p = p0 + k, ch = p / nsym, j = p % nsym;
+// pair k = blockIdx.y of the pass: p = p0 + k, chain p / nsym, symmetry
+// p % nsym.  Grids: (ceil(np^2 / 256), g), one thread per padded element.
+
+// T1[k] = Y1[chain] S_j:  T1[r, q] = sum_t Y1[r, cols(q)[t]] S_j[cols(q)[t], q]
+__global__ void rot_in_right(RotTab tb, const double* __restrict__ Y1b, int np, int nsym, int p0,
+                             double* __restrict__ T1) {
+  const int k = blockIdx.y, p = p0 + k, ch = p / nsym, j = p % nsym;
   const long nn = (long)np * np;
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nn) return;
   const int r = (int)(e / np), q = (int)(e % np);
   double v = 0.0;
-  if (r < n && q < n) {
-    const int br = t.mb[r], bc = t.mb[q];
-    const int dr = t.bd[br], dc = t.bd[bc];
-    const int* mr = t.bm + t.bmo[br];
-    const int* mc = t.bm + t.bmo[bc];
-    const double* Y = Y1b + (long)ch * nn;
-    for (int a = 0; a < dr; ++a) {
-      double u = 0.0;
-      for (int b = 0; b < dc; ++b) u += Y[(long)mr[a] * np + mc[b]] * s_at(t, j, mc[b], t.mp[q]);
-      v += s_at(t, j, mr[a], t.mp[r]) * u;
+  if (r < tb.n && q < tb.n) {
+    const double* y = Y1b + (long)ch * nn + (long)r * np;
+    const int* cq = tb.cols + (long)q * tb.dmax;
+    const double* s = tb.sc + ((long)j * tb.n + q) * tb.dmax;
+    const int d = tb.d[q];
+    for (int t = 0; t < d; ++t) v += y[cq[t]] * s[t];
+  }
+  T1[(long)k * nn + e] = v;
+}
+
+// Y1o[k] = S_j^T T1[k]:  Y1o[r, q] = sum_t S_j[cols(r)[t], r] T1[cols(r)[t], q];
+// R1o[k] = S_j^T R1[chain]
+__global__ void rot_in_left(RotTab tb, const double* __restrict__ T1,
+                            const double* __restrict__ R1b, int np, int nsym, int p0,
+                            double* __restrict__ R1o, double* __restrict__ Y1o) {
+  const int k = blockIdx.y, p = p0 + k, ch = p / nsym, j = p % nsym;
+  const long nn = (long)np * np;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  const int r = (int)(e / np), q = (int)(e % np);
+  double v = 0.0, w = 0.0;
+  if (r < tb.n) {
+    const int* cr = tb.cols + (long)r * tb.dmax;
+    const double* s = tb.sc + ((long)j * tb.n + r) * tb.dmax;
+    const int d = tb.d[r];
+    if (q < tb.n) {
+      const double* t1 = T1 + (long)k * nn + q;
+      for (int t = 0; t < d; ++t) v += s[t] * t1[(long)cr[t] * np];
     }
+    if (q == 0)
+      for (int t = 0; t < d; ++t) w += s[t] * R1b[(long)ch * np + cr[t]];
   }
   Y1o[(long)k * nn + e] = v;
-  if (e < np) {
-    double w = 0.0;
-    if (e < n) {
-      const int br = t.mb[e], dr = t.bd[br];
-      const int* mr = t.bm + t.bmo[br];
-      for (int a = 0; a < dr; ++a) w += s_at(t, j, mr[a], t.mp[e]) * R1b[(long)ch * np + mr[a]];
-    }
-    R1o[(long)k * np + e] = w;
+  if (q == 0) R1o[(long)k * np + r] = w;
+}
+
+// T1[k] = fY Z + (fY Z)^T of the pass's raw outer sums Z (32 x 32 tiles
+// through LDS, so that both reads are coalesced)
+__global__ void rot_sym(const double* __restrict__ zsum, const double* __restrict__ fy, int np,
+                        double* __restrict__ T1) {
+  __shared__ double tile[32][33];
+  const int k = blockIdx.z;
+  const long nn = (long)np * np;
+  const double* z = zsum + (long)k * nn;
+  const int bx = blockIdx.x * 32, by = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+  // tile of Z rows bx.., columns by.. (the transposed block)
+  for (int yy = ty; yy < 32; yy += 8) tile[yy][tx] = fy[bx + yy] * z[(long)(bx + yy) * np + by + tx];
+  __syncthreads();
+  for (int yy = ty; yy < 32; yy += 8) {
+    const int a = by + yy, b = bx + tx;
+    T1[(long)k * nn + (long)a * np + b] = fy[a] * z[(long)a * np + b] + tile[tx][yy];
   }
 }
 
-// add the pass's results, rotated back, to the outputs (unpadded n):
-//   f[ch]   += normf S_j f~            (f~ already divided by N_eff)
-//   d2v[ch] += normd S_j (fY Z + (fY Z)^T) S_j^T   (Z the raw outer sums)
-// one thread per output element, the pairs of the pass in order
-__global__ void rot_accumulate(RotTab t, const double* __restrict__ zsum,
-                               const double* __restrict__ fsum, const double* __restrict__ fy,
-                               int n, int np, int nsym, int p0, int g, double normf, double normd,
-                               double* __restrict__ f_all, double* __restrict__ d2v_all) {
-  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+// T2[k] = D~ S_j^T:  T2[a, q] = sum_t D~[a, cols(q)[t]] S_j[q, cols(q)[t]]
+__global__ void rot_out_right(RotTab tb, const double* __restrict__ T1, int np, int nsym, int p0,
+                              double* __restrict__ T2) {
+  const int k = blockIdx.y, p = p0 + k, j = p % nsym;
   const long nn = (long)np * np;
-  if (e < (long)n * n) {
-    const int r = (int)(e / n), q = (int)(e % n);
-    const int br = t.mb[r], bc = t.mb[q];
-    const int dr = t.bd[br], dc = t.bd[bc];
-    const int* mr = t.bm + t.bmo[br];
-    const int* mc = t.bm + t.bmo[bc];
-    for (int k = 0; k < g; ++k) {
-      const int p = p0 + k, ch = p / nsym, j = p % nsym;
-      const double* z = zsum + (long)k * nn;
-      double v = 0.0;
-      for (int a = 0; a < dr; ++a) {
-        const int ma = mr[a];
-        double u = 0.0;
-        for (int b = 0; b < dc; ++b) {
-          const int mq = mc[b];
-          const double d = fy[ma] * z[(long)ma * np + mq] + fy[mq] * z[(long)mq * np + ma];
-          u += d * s_at(t, j, q, b);
-        }
-        v += s_at(t, j, r, a) * u;
-      }
-      d2v_all[(long)ch * n * n + e] += normd * v;
-    }
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  const int a = (int)(e / np), q = (int)(e % np);
+  double v = 0.0;
+  if (a < tb.n && q < tb.n) {
+    const double* dt = T1 + (long)k * nn + (long)a * np;
+    const int* cq = tb.cols + (long)q * tb.dmax;
+    const double* s = tb.sr + ((long)j * tb.n + q) * tb.dmax;
+    const int d = tb.d[q];
+    for (int t = 0; t < d; ++t) v += dt[cq[t]] * s[t];
   }
-  if (e < n) {
-    const int br = t.mb[e], dr = t.bd[br];
-    const int* mr = t.bm + t.bmo[br];
-    for (int k = 0; k < g; ++k) {
-      const int p = p0 + k, ch = p / nsym, j = p % nsym;
+  T2[(long)k * nn + e] = v;
+}
+
+// d2v[chain] += normd S_j T2[k], f[chain] += normf S_j f~[k], the pairs of the
+// pass in order (one thread per output element of the unpadded outputs)
+__global__ void rot_out_left(RotTab tb, const double* __restrict__ T2,
+                             const double* __restrict__ fsum, int np, int nsym, int p0, int g,
+                             double normf, double normd, double* __restrict__ f_all,
+                             double* __restrict__ d2v_all) {
+  const int n = tb.n;
+  const long nn = (long)np * np;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  const int r = (int)(e / n), q = (int)(e % n);
+  const int* cr = tb.cols + (long)r * tb.dmax;
+  const int d = tb.d[r];
+  for (int k = 0; k < g; ++k) {
+    const int p = p0 + k, ch = p / nsym, j = p % nsym;
+    const double* s = tb.sr + ((long)j * n + r) * tb.dmax;
+    const double* t2 = T2 + (long)k * nn + q;
+    double v = 0.0;
+    for (int t = 0; t < d; ++t) v += s[t] * t2[(long)cr[t] * np];
+    d2v_all[(long)ch * n * n + e] += normd * v;
+    if (q == 0) {
       double w = 0.0;
-      for (int a = 0; a < dr; ++a) w += s_at(t, j, (int)e, a) * fsum[(long)k * np + mr[a]];
-      f_all[(long)ch * n + e] += normf * w;
+      for (int t = 0; t < d; ++t) w += s[t] * fsum[(long)k * np + cr[t]];
+      f_all[(long)ch * n + r] += normf * w;
     }
   }
 }
@@ -161,7 +193,7 @@ int run_pa_rot(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2
                hipStream_t s) {
   const int n = c->n, np = c->np, ns = c->rot_nsym;
   const long nn = (long)np * np;
-  const RotTab t{c->rt_mb, c->rt_mp, c->rt_bd, c->rt_bmo, c->rt_bm, c->rt_bso, c->rt_syms};
+  const RotTab tb{c->rt_cols, c->rt_d, c->rt_sr, c->rt_sc, c->rt_dmax, n};
   hipLaunchKernelGGL(rot_copy, dim3(64), dim3(256), 0, s, c->R1, (long)nc * np, c->R1_base);
   hipLaunchKernelGGL(rot_copy, dim3(1024), dim3(256), 0, s, c->Y1, (long)nc * nn, c->Y1_base);
   hipLaunchKernelGGL(rot_zero, dim3(64), dim3(256), 0, s, (long)nc * n, f_dev);
@@ -169,17 +201,23 @@ int run_pa_rot(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2
   TDS_CHECK_HIP(hipGetLastError());
   const int pairs = nc * ns;
   const double normf = 1.0 / ns, normd = 1.0 / (c->n_eff * ns);
+  const unsigned eb = (unsigned)((nn + 255) / 256);
   for (int p0 = 0; p0 < pairs; p0 += kMaxChains) {
     const int g = pairs - p0 < kMaxChains ? pairs - p0 : kMaxChains;
-    hipLaunchKernelGGL(rot_inputs, dim3((unsigned)((nn + 255) / 256), g), dim3(256), 0, s, t,
-                       c->R1_base, c->Y1_base, n, np, ns, p0, c->R1, c->Y1);
+    hipLaunchKernelGGL(rot_in_right, dim3(eb, g), dim3(256), 0, s, tb, c->Y1_base, np, ns, p0,
+                       c->rot_t1);
+    hipLaunchKernelGGL(rot_in_left, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t1, c->R1_base, np,
+                       ns, p0, c->R1, c->Y1);
     TDS_CHECK_HIP(hipGetLastError());
     const int rc = prec == PREC_FP64 ? launch_pa_fp64(c, g, apply_d4, s, nullptr)
                                      : launch_pa_split(c, g, apply_d4, prec, s, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL(rot_accumulate, dim3((unsigned)(((long)n * n + 255) / 256)), dim3(256), 0, s,
-                       t, c->d2v_out, c->f_out, c->fy, n, np, ns, p0, g, normf, normd, f_dev,
-                       d2v_dev);
+    hipLaunchKernelGGL(rot_sym, dim3(np / 32, np / 32, g), dim3(256), 0, s, c->d2v_out, c->fy, np,
+                       c->rot_t1);
+    hipLaunchKernelGGL(rot_out_right, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t1, np, ns, p0,
+                       c->rot_t2);
+    hipLaunchKernelGGL(rot_out_left, dim3((unsigned)(((long)n * n + 255) / 256)), dim3(256), 0, s,
+                       tb, c->rot_t2, c->f_out, np, ns, p0, g, normf, normd, f_dev, d2v_dev);
     TDS_CHECK_HIP(hipGetLastError());
   }
   return 0;
