@@ -192,14 +192,15 @@ struct Ctx {
   bool rotate = false;
   int rot_nsym = 1;
   int rt_dmax = 0;             // largest block
-  int* rt_cols = nullptr;      // [n, dmax] modes of each mode's block
+  int* rt_cols = nullptr;      // [dmax, n] modes of each mode's block (position-major)
   int* rt_d = nullptr;         // [n] size of each mode's block
-  double* rt_sr = nullptr;     // [rot_nsym, n, dmax] S_j[m, cols(m)[t]]   (row m of S_j)
-  double* rt_sc = nullptr;     // [rot_nsym, n, dmax] S_j[cols(m)[t], m]   (column m)
+  double* rt_sr = nullptr;     // [rot_nsym, dmax, n] S_j[m, cols(m)[t]]   (row m of S_j)
+  double* rt_sc = nullptr;     // [rot_nsym, dmax, n] S_j[cols(m)[t], m]   (column m)
   double* R1_base = nullptr;   // [kMaxChains, np] unrotated padded inputs of a call
   double* Y1_base = nullptr;   // [kMaxChains, np, np]
   double* rot_t1 = nullptr;    // [kMaxChains, np, np] rotation intermediates
   double* rot_t2 = nullptr;
+  double* rot_f = nullptr;     // [kMaxChains, np]
 };
 
 int ensure_split(Ctx* c, int prec);

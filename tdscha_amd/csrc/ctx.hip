@@ -251,11 +251,11 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
       for (int a = 0; a < d; ++a) {
         const int m = bm[a];
         dd[m] = d;
-        for (int t = 0; t < d; ++t) cols[(size_t)m * dmax + t] = bm[t];
+        for (int t = 0; t < d; ++t) cols[(size_t)t * n_modes + m] = bm[t];
         for (int j = 0; j < n_syms; ++j)
           for (int t = 0; t < d; ++t) {
-            sr[((size_t)j * n_modes + m) * dmax + t] = block_syms[b][((size_t)j * d + a) * d + t];
-            sc[((size_t)j * n_modes + m) * dmax + t] = block_syms[b][((size_t)j * d + t) * d + a];
+            sr[((size_t)j * dmax + t) * n_modes + m] = block_syms[b][((size_t)j * d + a) * d + t];
+            sc[((size_t)j * dmax + t) * n_modes + m] = block_syms[b][((size_t)j * d + t) * d + a];
           }
       }
     }
@@ -267,7 +267,8 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
          hipMalloc((void**)&c->R1_base, sizeof(double) * kMaxChains * np) == hipSuccess &&
          hipMalloc((void**)&c->Y1_base, sizeof(double) * kMaxChains * (long)np * np) == hipSuccess &&
          hipMalloc((void**)&c->rot_t1, sizeof(double) * kMaxChains * (long)np * np) == hipSuccess &&
-         hipMalloc((void**)&c->rot_t2, sizeof(double) * kMaxChains * (long)np * np) == hipSuccess;
+         hipMalloc((void**)&c->rot_t2, sizeof(double) * kMaxChains * (long)np * np) == hipSuccess &&
+         hipMalloc((void**)&c->rot_f, sizeof(double) * kMaxChains * np) == hipSuccess;
     ok = ok && hipMemcpy(c->rt_cols, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(c->rt_d, dd.data(), sizeof(int) * n_modes, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(c->rt_sr, sr.data(), sizeof(double) * sr.size(), hipMemcpyHostToDevice) == hipSuccess;
@@ -304,7 +305,7 @@ void tdscha_ctx_destroy(tdscha_ctx* ctx) {
                   c->stage_in, c->stage_out, c->f_fin, c->d2v_fin, c->f_proj, c->epi,
                   c->coef32, c->wpart, c->zpart32, c->fpart_s, c->sbsc, c->pmax,
                   c->rt_cols, c->rt_d, c->rt_sr, c->rt_sc, c->R1_base, c->Y1_base, c->rot_t1,
-                  c->rot_t2};
+                  c->rot_t2, c->rot_f};
   for (void* p : ptrs)
     if (p) hipFree(p);
   for (auto& s : c->split) {

@@ -29,13 +29,22 @@ namespace tdscha {
 int launch_pa_fp64(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* ev);
 int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s, hipEvent_t* ev);
 
+// position-major tables, so that neighbouring modes (lanes) read neighbouring
+// entries: entry t of mode m at [t * n + m]
 struct RotTab {
-  const int* cols;      // [n, dmax]
+  const int* cols;      // [dmax, n]
   const int* d;         // [n]
-  const double* sr;     // [nsym, n, dmax] S_j[m, cols(m)[t]]
-  const double* sc;     // [nsym, n, dmax] S_j[cols(m)[t], m]
+  const double* sr;     // [nsym, dmax, n] S_j[m, cols(m)[t]]
+  const double* sc;     // [nsym, dmax, n] S_j[cols(m)[t], m]
   int dmax;
   int n;
+  __device__ __forceinline__ int col(int m, int t) const { return cols[(long)t * n + m]; }
+  __device__ __forceinline__ double row_s(int j, int m, int t) const {
+    return sr[((long)j * dmax + t) * n + m];
+  }
+  __device__ __forceinline__ double col_s(int j, int m, int t) const {
+    return sc[((long)j * dmax + t) * n + m];
+  }
 };
 
 // the configurations, padded to [Mp, np] (zero rows and columns beyond)
@@ -79,10 +88,8 @@ __global__ void rot_in_right(RotTab tb, const double* __restrict__ Y1b, int np, 
   double v = 0.0;
   if (r < tb.n && q < tb.n) {
     const double* y = Y1b + (long)ch * nn + (long)r * np;
-    const int* cq = tb.cols + (long)q * tb.dmax;
-    const double* s = tb.sc + ((long)j * tb.n + q) * tb.dmax;
     const int d = tb.d[q];
-    for (int t = 0; t < d; ++t) v += y[cq[t]] * s[t];
+    for (int t = 0; t < d; ++t) v += y[tb.col(q, t)] * tb.col_s(j, q, t);
   }
   T1[(long)k * nn + e] = v;
 }
@@ -99,15 +106,13 @@ __global__ void rot_in_left(RotTab tb, const double* __restrict__ T1,
   const int r = (int)(e / np), q = (int)(e % np);
   double v = 0.0, w = 0.0;
   if (r < tb.n) {
-    const int* cr = tb.cols + (long)r * tb.dmax;
-    const double* s = tb.sc + ((long)j * tb.n + r) * tb.dmax;
     const int d = tb.d[r];
     if (q < tb.n) {
       const double* t1 = T1 + (long)k * nn + q;
-      for (int t = 0; t < d; ++t) v += s[t] * t1[(long)cr[t] * np];
+      for (int t = 0; t < d; ++t) v += tb.col_s(j, r, t) * t1[(long)tb.col(r, t) * np];
     }
     if (q == 0)
-      for (int t = 0; t < d; ++t) w += s[t] * R1b[(long)ch * np + cr[t]];
+      for (int t = 0; t < d; ++t) w += tb.col_s(j, r, t) * R1b[(long)ch * np + tb.col(r, t)];
   }
   Y1o[(long)k * nn + e] = v;
   if (q == 0) R1o[(long)k * np + r] = w;
@@ -143,39 +148,48 @@ __global__ void rot_out_right(RotTab tb, const double* __restrict__ T1, int np, 
   double v = 0.0;
   if (a < tb.n && q < tb.n) {
     const double* dt = T1 + (long)k * nn + (long)a * np;
-    const int* cq = tb.cols + (long)q * tb.dmax;
-    const double* s = tb.sr + ((long)j * tb.n + q) * tb.dmax;
     const int d = tb.d[q];
-    for (int t = 0; t < d; ++t) v += dt[cq[t]] * s[t];
+    for (int t = 0; t < d; ++t) v += dt[tb.col(q, t)] * tb.row_s(j, q, t);
   }
   T2[(long)k * nn + e] = v;
 }
 
-// d2v[chain] += normd S_j T2[k], f[chain] += normf S_j f~[k], the pairs of the
-// pass in order (one thread per output element of the unpadded outputs)
+// T1[k] = S_j T2[k] (padded np), F1[k] = S_j f~[k]
 __global__ void rot_out_left(RotTab tb, const double* __restrict__ T2,
-                             const double* __restrict__ fsum, int np, int nsym, int p0, int g,
-                             double normf, double normd, double* __restrict__ f_all,
-                             double* __restrict__ d2v_all) {
-  const int n = tb.n;
+                             const double* __restrict__ fsum, int np, int nsym, int p0,
+                             double* __restrict__ T1, double* __restrict__ F1) {
+  const int k = blockIdx.y, j = (p0 + k) % nsym;
+  const long nn = (long)np * np;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nn) return;
+  const int r = (int)(e / np), q = (int)(e % np);
+  double v = 0.0, w = 0.0;
+  if (r < tb.n) {
+    const int d = tb.d[r];
+    if (q < tb.n) {
+      const double* t2 = T2 + (long)k * nn + q;
+      for (int t = 0; t < d; ++t) v += tb.row_s(j, r, t) * t2[(long)tb.col(r, t) * np];
+    }
+    if (q == 0)
+      for (int t = 0; t < d; ++t) w += tb.row_s(j, r, t) * fsum[(long)k * np + tb.col(r, t)];
+  }
+  T1[(long)k * nn + e] = v;
+  if (q == 0) F1[(long)k * np + r] = w;
+}
+
+// d2v[chain] += normd T1[k], f[chain] += normf F1[k] for the pairs of the pass
+// in order (one thread per element of the unpadded outputs)
+__global__ void rot_accumulate(const double* __restrict__ T1, const double* __restrict__ F1,
+                               int n, int np, int nsym, int p0, int g, double normf, double normd,
+                               double* __restrict__ f_all, double* __restrict__ d2v_all) {
   const long nn = (long)np * np;
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)n * n) return;
   const int r = (int)(e / n), q = (int)(e % n);
-  const int* cr = tb.cols + (long)r * tb.dmax;
-  const int d = tb.d[r];
   for (int k = 0; k < g; ++k) {
-    const int p = p0 + k, ch = p / nsym, j = p % nsym;
-    const double* s = tb.sr + ((long)j * n + r) * tb.dmax;
-    const double* t2 = T2 + (long)k * nn + q;
-    double v = 0.0;
-    for (int t = 0; t < d; ++t) v += s[t] * t2[(long)cr[t] * np];
-    d2v_all[(long)ch * n * n + e] += normd * v;
-    if (q == 0) {
-      double w = 0.0;
-      for (int t = 0; t < d; ++t) w += s[t] * fsum[(long)k * np + cr[t]];
-      f_all[(long)ch * n + r] += normf * w;
-    }
+    const int ch = (p0 + k) / nsym;
+    d2v_all[(long)ch * n * n + e] += normd * T1[(long)k * nn + (long)r * np + q];
+    if (e < n) f_all[(long)ch * n + e] += normf * F1[(long)k * np + e];
   }
 }
 
@@ -216,8 +230,10 @@ int run_pa_rot(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2
                        c->rot_t1);
     hipLaunchKernelGGL(rot_out_right, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t1, np, ns, p0,
                        c->rot_t2);
-    hipLaunchKernelGGL(rot_out_left, dim3((unsigned)(((long)n * n + 255) / 256)), dim3(256), 0, s,
-                       tb, c->rot_t2, c->f_out, np, ns, p0, g, normf, normd, f_dev, d2v_dev);
+    hipLaunchKernelGGL(rot_out_left, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t2, c->f_out, np, ns,
+                       p0, c->rot_t1, c->rot_f);
+    hipLaunchKernelGGL(rot_accumulate, dim3((unsigned)(((long)n * n + 255) / 256)), dim3(256), 0, s,
+                       c->rot_t1, c->rot_f, n, np, ns, p0, g, normf, normd, f_dev, d2v_dev);
     TDS_CHECK_HIP(hipGetLastError());
   }
   return 0;
