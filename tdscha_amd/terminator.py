@@ -10,6 +10,9 @@ coefficients and the reference's continued fraction (with its terminator on
 the predicted asymptote) gives the Green function, so fewer L applications
 reach a given spectrum accuracy.
 
+One ensemble is trained per chain length K (``TerminatorFamily``): the
+shorter the chain, the more the learned tail gains over the constant one.
+
 An ensemble of independently trained networks gives a spread; where the
 spread of the predicted asymptote exceeds a threshold fixed at training time,
 or the chain is not exactly K steps long, the standard constant terminator is
@@ -280,6 +283,56 @@ class LearnedTerminator:
             for n in self._nets:
                 n.close()
             self._nets = None
+
+
+PACKAGED_LENGTHS = (20, 30, 40)
+
+
+def packaged_path(K):
+    """The trained ensemble for K-step chains of the 16-atom rock-salt models."""
+    import os
+    return os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                        "terminator_rocksalt16_K%d.npz" % K)
+
+
+class TerminatorFamily:
+    """Learned terminators for several chain lengths, one trained ensemble per
+    K.  The learned closure gains most on short chains, so the family lets a
+    caller stop the recurrence early: a chain of K steps is closed by the K
+    ensemble, a chain of a length without one by the standard terminator.
+    ``last_used`` tells which closed the last chain."""
+
+    def __init__(self, terms):
+        self.terms = {t.K: t for t in terms}
+        self.last_used = None
+
+    @classmethod
+    def load(cls, paths):
+        return cls([LearnedTerminator.load(p) for p in paths])
+
+    @classmethod
+    def packaged(cls):
+        return cls.load([packaged_path(K) for K in PACKAGED_LENGTHS])
+
+    @property
+    def lengths(self):
+        return sorted(self.terms)
+
+    def green_function(self, a, b, w, smearing, perturbation_modulus=1.0, last_average=1):
+        term = self.terms.get(len(a))
+        if term is None:
+            self.last_used = "standard"
+            return continued_fraction(np.asarray(a, np.float64), np.asarray(b, np.float64),
+                                      np.asarray(b, np.float64), w, True, use_terminator=True,
+                                      last_average=last_average, smearing=smearing,
+                                      perturbation_modulus=perturbation_modulus)
+        g = term.green_function(a, b, w, smearing, perturbation_modulus, last_average)
+        self.last_used = term.last_used
+        return g
+
+    def close(self):
+        for t in self.terms.values():
+            t.close()
 
 
 # --------------------------------------------------------------------------

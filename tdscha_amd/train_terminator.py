@@ -1,7 +1,8 @@
 """Generate Lanczos chains on the GPU and train the learned terminator.
 
-    python -m tdscha_amd.train_terminator --out-dir DIR [--n-train 5000]
-        [--n-heldout 200] [--train-steps 400] [--ref-steps 1000]
+    python -m tdscha_amd.train_terminator --out-dir DIR [--K 20 30 40]
+        [--n-train 5000] [--n-heldout 200] [--train-steps 400] [--ref-steps 1000]
+        [--save-chains FILE | --chains FILE]
 
 Training data: rock-salt 2x2x2 spring models (16 atoms, 45 modes) with
 k3 ~ U(-4, 0) eV/A^3, k4 ~ U(2, 10) eV/A^4, T ~ U(50, 600) K, one seeded
@@ -11,8 +12,11 @@ reorthogonalisation, FP64) run as one batch by the library's device Lanczos.
 Ensembles are built by CPU worker processes started before the GPU is
 touched.  Held-out potentials come from a separate seed stream and are run
 to ``--ref-steps``; their spectra are the reference for the standard and the learned
-terminator at K steps.  Writes DIR/terminator_rocksalt16.npz (the trained
-ensemble) and DIR/terminator_rocksalt16.json (held-out report).
+terminator at K steps.  One ensemble is trained per K on the same chains.
+Writes DIR/terminator_rocksalt16_K<K>.npz (the trained ensemble) and
+DIR/terminator_rocksalt16_K<K>.json (held-out report) for every K.
+``--chains`` trains from chains written earlier by ``--save-chains``
+instead of generating them (the GPU still trains).
 """
 from __future__ import annotations
 
@@ -146,33 +150,43 @@ def main():
     ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--ref-steps", type=int, default=1000)
     ap.add_argument("--configs", type=int, default=2000)
-    ap.add_argument("--K", type=int, default=40)
+    ap.add_argument("--K", type=int, nargs="+", default=[20, 30, 40])
     ap.add_argument("--members", type=int, default=4)
     ap.add_argument("--stage1-epochs", type=int, default=300)
     ap.add_argument("--stage2-epochs", type=int, default=150)
     ap.add_argument("--workers", type=int, default=12)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--save-chains", default=None, help="also write the raw chains (npz)")
+    ap.add_argument("--chains", default=None,
+                    help="train from chains written by --save-chains instead of generating them")
     args = ap.parse_args()
     os.makedirs(args.out_dir, exist_ok=True)
-    os.environ["OMP_NUM_THREADS"] = "1"
-    os.environ["OPENBLAS_NUM_THREADS"] = "1"
-    ctx = mp.get_context("spawn")
-    pool = ctx.Pool(args.workers)   # started before anything touches the GPU
 
-    from tdscha_amd import terminator as TM
-    n_pot = (args.n_train + 2) // 3
     t0 = time.time()
-    ptrain = potential_params(n_pot, args.seed)
-    A, B, lens = run_chains(pool, ptrain, 100000, args.configs, args.train_steps, "train")
-    pheld = potential_params(args.n_heldout, args.seed + 7919)
-    Ah, Bh, lh = run_chains(pool, pheld, 900000, args.configs, args.ref_steps, "held-out")
-    pool.close()
-    pool.join()
+    if args.chains:
+        d = np.load(args.chains, allow_pickle=False)
+        A, B, lens, Ah, Bh, lh = (d[k] for k in ("A", "B", "lens", "Ah", "Bh", "lh"))
+        args.train_steps, args.ref_steps = A.shape[-1], Ah.shape[-1]
+        args.n_heldout = Ah.shape[0]
+        log("chains from %s: %d training and %d held-out potentials" % (
+            args.chains, A.shape[0], Ah.shape[0]))
+    else:
+        os.environ["OMP_NUM_THREADS"] = "1"
+        os.environ["OPENBLAS_NUM_THREADS"] = "1"
+        ctx = mp.get_context("spawn")
+        pool = ctx.Pool(args.workers)   # started before anything touches the GPU
+        n_pot = (args.n_train + 2) // 3
+        ptrain = potential_params(n_pot, args.seed)
+        A, B, lens = run_chains(pool, ptrain, 100000, args.configs, args.train_steps, "train")
+        pheld = potential_params(args.n_heldout, args.seed + 7919)
+        Ah, Bh, lh = run_chains(pool, pheld, 900000, args.configs, args.ref_steps, "held-out")
+        pool.close()
+        pool.join()
+        if args.save_chains:
+            np.savez(args.save_chains, A=A, B=B, lens=lens, ptrain=np.array(ptrain), Ah=Ah,
+                     Bh=Bh, lh=lh, pheld=np.array(pheld))
     t_gen = time.time() - t0
-    if args.save_chains:
-        np.savez(args.save_chains, A=A, B=B, lens=lens, ptrain=np.array(ptrain), Ah=Ah, Bh=Bh,
-                 lh=lh, pheld=np.array(pheld))
+    from tdscha_amd import terminator as TM
 
     # training set: full-length chains, the first n_train of them
     a_tr = A.reshape(-1, args.train_steps)
@@ -182,28 +196,38 @@ def main():
     log("training chains: %d (generation %.1fs)" % (len(a_tr), t_gen))
     w = np.linspace(0.0, 600.0, 2000) / RY_TO_CM
     eta = 5.0 / RY_TO_CM
-    t1 = time.time()
-    term, curves = TM.train_terminator(a_tr, b_tr, w, eta, K=args.K, n_members=args.members,
-                                       stage1_epochs=args.stage1_epochs,
-                                       stage2_epochs=args.stage2_epochs, seed=args.seed, log=log)
-    t_train = time.time() - t1
-    log("trained %d members in %.1fs; variance threshold %.4g" % (
-        args.members, t_train, term.var_threshold))
-    rep = evaluate(term, Ah, Bh, lh, w, eta, args.K, 120)
-    rep.update({
-        "K": args.K, "n_train_chains": int(len(a_tr)), "train_steps": args.train_steps,
-        "ref_steps": args.ref_steps, "n_heldout_potentials": args.n_heldout,
-        "configs_per_ensemble": args.configs, "members": args.members,
-        "generation_s": t_gen, "training_s": t_train, "var_threshold": term.var_threshold,
-        "grid": "2000 points 0-600 cm^-1, smearing 5 cm^-1",
-        "curves": curves,
-    })
-    term.info = {k: v for k, v in rep.items() if k not in ("per_chain", "curves")}
-    term.save(os.path.join(args.out_dir, "terminator_rocksalt16.npz"))
-    with open(os.path.join(args.out_dir, "terminator_rocksalt16.json"), "w") as fp:
-        json.dump(rep, fp)
-    summ = {k: v for k, v in rep.items() if k not in ("per_chain", "curves")}
-    log(json.dumps(summ, indent=1))
+    summary = {}
+    for K in args.K:
+        t1 = time.time()
+        term, curves = TM.train_terminator(a_tr, b_tr, w, eta, K=K, n_members=args.members,
+                                           stage1_epochs=args.stage1_epochs,
+                                           stage2_epochs=args.stage2_epochs, seed=args.seed,
+                                           log=log)
+        t_train = time.time() - t1
+        log("K = %d: trained %d members in %.1fs; variance threshold %.4g" % (
+            K, args.members, t_train, term.var_threshold))
+        rep = evaluate(term, Ah, Bh, lh, w, eta, K, 120)
+        rep.update({
+            "K": K, "n_train_chains": int(len(a_tr)), "train_steps": args.train_steps,
+            "ref_steps": args.ref_steps, "n_heldout_potentials": args.n_heldout,
+            "configs_per_ensemble": args.configs, "members": args.members,
+            "chains_from": args.chains or "generated", "generation_s": t_gen,
+            "training_s": t_train, "var_threshold": term.var_threshold,
+            "saving_median": 1.0 - K / rep["median_steps_standard_needs"],
+            "saving_mean": 1.0 - K / rep["mean_steps_standard_needs"],
+            "grid": "2000 points 0-600 cm^-1, smearing 5 cm^-1",
+            "curves": curves,
+        })
+        term.info = {k: v for k, v in rep.items() if k not in ("per_chain", "curves")}
+        term.save(os.path.join(args.out_dir, "terminator_rocksalt16_K%d.npz" % K))
+        term.close()
+        with open(os.path.join(args.out_dir, "terminator_rocksalt16_K%d.json" % K), "w") as fp:
+            json.dump(rep, fp)
+        summary[K] = term.info
+        log(json.dumps(term.info, indent=1))
+    log(json.dumps({K: {k: v[k] for k in ("median_err_standard", "median_err_learned",
+                                          "median_steps_standard_needs", "saving_median",
+                                          "saving_mean")} for K, v in summary.items()}))
     return 0
 
 

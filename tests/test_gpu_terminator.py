@@ -17,8 +17,6 @@ pytestmark = pytest.mark.gpu
 RY_TO_CM = 109737.31568527
 K = 40
 DIMS = [2 * K, 128, 128, 128, 10]
-DATA = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tdscha_amd",
-                    "data", "terminator_rocksalt16.npz")
 
 
 def _net(params, max_batch=64):
@@ -178,19 +176,16 @@ def test_spectral_training_reduces_loss():
     net.close()
 
 
-@pytest.mark.skipif(not os.path.exists(DATA), reason="trained terminator not present")
-def test_learned_terminator_gains_over_standard_on_heldout_potentials():
-    """Held-out potentials (seeds outside the training streams): spectra of
-    40-step chains closed by the learned tail versus the standard constant
-    terminator, scored against 1000-step FP64 chains of the same potential."""
+@pytest.fixture(scope="module")
+def heldout_chains():
+    """Held-out potentials (seeds outside the training streams): three
+    1000-step FP64 chains each and the spectra they give."""
     from tdscha_amd.lanczos import Lanczos, continued_fraction
     from tdscha_amd.rocksalt import Ensemble, RockSalt
-    from tdscha_amd.terminator import LearnedTerminator
-    term = LearnedTerminator.load(DATA)
     rng = np.random.default_rng(31337)
     w = np.linspace(0.0, 600.0, 2000) / RY_TO_CM
     eta = 5.0 / RY_TO_CM
-    e_std, e_l, used = [], [], []
+    out = []
     for i in range(8):
         k3, k4, T = rng.uniform(-4, 0), rng.uniform(2, 10), rng.uniform(50, 600)
         ens = Ensemble(RockSalt(L=2, k3=k3, k4=k4), T, 2000, seed=7_000_000 + i)
@@ -204,17 +199,94 @@ def test_learned_terminator_gains_over_standard_on_heldout_potentials():
         lanc.close()
         for a, b in chains:
             ref = -np.imag(continued_fraction(a, b, b, w, True, smearing=eta))
-            sc = np.abs(ref).max()
-            std = -np.imag(continued_fraction(a[:K], b[:K], b[:K], w, True, smearing=eta))
-            lrn = -np.imag(term.green_function(a[:K], b[:K], w, eta))
-            used.append(term.last_used == "learned")
-            e_std.append(np.abs(std - ref).max() / sc)
-            e_l.append(np.abs(lrn - ref).max() / sc)
+            out.append((a, b, ref))
+    return out, w, eta
+
+
+def _closures(term, chains, w, eta, K, k_max=None):
+    """Per chain: the L-inf spectrum errors (relative to the 1000-step
+    spectrum) of the standard and the learned closure after K steps, whether
+    the learned tail was used, and (with k_max) the number of steps the
+    standard terminator needs to reach the learned closure's error."""
+    from tdscha_amd.lanczos import continued_fraction
+    rows = []
+    for a, b, ref in chains:
+        sc = np.abs(ref).max()
+
+        def err_std(k):
+            s = -np.imag(continued_fraction(a[:k], b[:k], b[:k], w, True, smearing=eta))
+            return np.abs(s - ref).max() / sc
+
+        lrn = -np.imag(term.green_function(a[:K], b[:K], w, eta))
+        e_l = np.abs(lrn - ref).max() / sc
+        k_eq = None
+        if k_max is not None:
+            k_eq = next((k for k in range(K, k_max + 1) if err_std(k) <= e_l), k_max + 1)
+        rows.append((err_std(K), e_l, term.last_used == "learned", k_eq))
+    return [np.array(c) for c in zip(*rows)]
+
+
+def _data(K):
+    from tdscha_amd.terminator import packaged_path
+    return packaged_path(K)
+
+
+@pytest.mark.skipif(not os.path.exists(_data(K)), reason="trained terminator not present")
+def test_learned_terminator_gains_over_standard_on_heldout_potentials(heldout_chains):
+    """40-step chains closed by the learned tail versus the standard constant
+    terminator, scored against 1000-step FP64 chains of the same potential."""
+    from tdscha_amd.terminator import LearnedTerminator
+    chains, w, eta = heldout_chains
+    term = LearnedTerminator.load(_data(K))
+    e_std, e_l, used, _ = _closures(term, chains, w, eta, K)
     term.close()
-    e_std, e_l = np.array(e_std), np.array(e_l)
     ratio = np.exp(np.mean(np.log(e_l / e_std)))
     print("standard median %.3e learned median %.3e geo-mean ratio %.3f wins %d/%d used %d" % (
         np.median(e_std), np.median(e_l), ratio, (e_l < e_std).sum(), len(e_l), sum(used)))
     assert sum(used) >= len(used) // 2
     assert ratio < 0.9
     assert np.mean(e_l < e_std) >= 0.6
+
+
+@pytest.mark.skipif(not os.path.exists(_data(20)), reason="trained terminator not present")
+def test_short_chain_closure_saves_a_quarter_of_the_L_applications(heldout_chains):
+    """20-step chains closed by the K = 20 ensemble: the standard terminator
+    needs a median of at least 27 steps (25% more L applications than 20) to
+    reach the same spectrum error on the held-out potentials."""
+    from tdscha_amd.terminator import LearnedTerminator
+    chains, w, eta = heldout_chains
+    term = LearnedTerminator.load(_data(20))
+    e_std, e_l, used, k_eq = _closures(term, chains, w, eta, 20, k_max=120)
+    term.close()
+    saving = 1.0 - 20.0 / np.median(k_eq)
+    print("K = 20: standard median %.3e learned median %.3e; standard needs median %d / mean "
+          "%.1f steps (saving %.0f%%), used %d/%d" % (np.median(e_std), np.median(e_l),
+                                                     np.median(k_eq), np.mean(k_eq),
+                                                     100 * saving, sum(used), len(used)))
+    assert sum(used) >= len(used) // 2
+    assert np.median(k_eq) >= 27
+
+
+@pytest.mark.skipif(not all(os.path.exists(_data(k)) for k in (20, 30, 40)),
+                    reason="trained terminators not present")
+def test_terminator_family_dispatches_on_chain_length(heldout_chains):
+    """The family closes a chain with the ensemble trained for its length and
+    any other length with the reference's standard terminator."""
+    from tdscha_amd.lanczos import continued_fraction
+    from tdscha_amd.terminator import LearnedTerminator, TerminatorFamily
+    chains, w, eta = heldout_chains
+    a, b, _ = chains[0]
+    fam = TerminatorFamily.packaged()
+    assert fam.lengths == [20, 30, 40]
+    for k in (20, 30, 40):
+        single = LearnedTerminator.load(_data(k))
+        g1 = fam.green_function(a[:k], b[:k], w, eta)
+        g2 = single.green_function(a[:k], b[:k], w, eta)
+        assert fam.last_used == single.last_used
+        assert np.array_equal(g1, g2)
+        single.close()
+    g = fam.green_function(a[:25], b[:25], w, eta)
+    assert fam.last_used == "standard"
+    gs = continued_fraction(a[:25], b[:25], b[:25], w, True, smearing=eta)
+    assert np.array_equal(g, gs)
+    fam.close()
