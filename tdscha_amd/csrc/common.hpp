@@ -196,6 +196,10 @@ struct Ctx {
   int* rt_d = nullptr;         // [n] size of each mode's block
   double* rt_sr = nullptr;     // [rot_nsym, dmax, n] S_j[m, cols(m)[t]]   (row m of S_j)
   double* rt_sc = nullptr;     // [rot_nsym, dmax, n] S_j[cols(m)[t], m]   (column m)
+  int* rt_bs = nullptr;        // [n] first mode of each mode's block (panel kernels)
+  int* rt_pstart = nullptr;    // [rt_npanel + 1] first mode of each panel, then np
+  int rt_npanel = 0, rt_nreal = 0;
+  bool rt_panels = false;      // every block consecutive modes, at most 24: panel kernels
   double* R1_base = nullptr;   // [kMaxChains, np] unrotated padded inputs of a call
   double* Y1_base = nullptr;   // [kMaxChains, np, np]
   double* rot_t1 = nullptr;    // [kMaxChains, np, np] rotation intermediates

@@ -21,6 +21,8 @@ int launch_pa_split(Ctx* c, int nc, int apply_d4, int prec, hipStream_t s, hipEv
 int set_ksplit_split(Ctx* c, int ks);
 int default_ksplit_split(Ctx* c);
 int rot_setup(Ctx* c, const double* X_dev, const double* Y_dev, const double* rho_dev);
+bool rot_panels_fit(int dmax);
+int rot_panels(const int* bs, const int* d, int n, int np, int* start, int* nreal);
 int run_pa_rot(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2v_dev,
                hipStream_t s);
 
@@ -243,14 +245,27 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
     // and column of every S_j restricted to them
     int dmax = 0;
     for (int b = 0; b < n_blocks; ++b) dmax = std::max(dmax, block_dim[b]);
-    std::vector<int> cols((size_t)n_modes * dmax, 0), dd(n_modes, 0);
+    std::vector<int> cols((size_t)n_modes * dmax, 0), dd(n_modes, 0), bs(n_modes, 0);
+    // blocks of consecutive modes in ascending order take the panel kernels
+    bool consecutive = true;
+    for (int b = 0; b < n_blocks; ++b) {
+      const int* bm = block_modes + bmoff[b];
+      for (int a = 0; a < block_dim[b]; ++a) {
+        bs[bm[a]] = bm[0];
+        dd[bm[a]] = block_dim[b];
+        consecutive = consecutive && bm[a] == bm[0] + a;
+      }
+    }
+    c->rt_panels = consecutive && rot_panels_fit(dmax);
+    std::vector<int> pstart(n_modes + 2, 0);
+    if (c->rt_panels)
+      c->rt_npanel = rot_panels(bs.data(), dd.data(), n_modes, np, pstart.data(), &c->rt_nreal);
     std::vector<double> sr((size_t)n_syms * n_modes * dmax, 0.0), sc(sr.size(), 0.0);
     for (int b = 0; b < n_blocks; ++b) {
       const int d = block_dim[b];
       const int* bm = block_modes + bmoff[b];
       for (int a = 0; a < d; ++a) {
         const int m = bm[a];
-        dd[m] = d;
         for (int t = 0; t < d; ++t) cols[(size_t)t * n_modes + m] = bm[t];
         for (int j = 0; j < n_syms; ++j)
           for (int t = 0; t < d; ++t) {
@@ -262,6 +277,8 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
     c->rt_dmax = dmax;
     ok = hipMalloc((void**)&c->rt_cols, sizeof(int) * cols.size()) == hipSuccess &&
          hipMalloc((void**)&c->rt_d, sizeof(int) * n_modes) == hipSuccess &&
+         hipMalloc((void**)&c->rt_bs, sizeof(int) * n_modes) == hipSuccess &&
+         hipMalloc((void**)&c->rt_pstart, sizeof(int) * (n_modes + 2)) == hipSuccess &&
          hipMalloc((void**)&c->rt_sr, sizeof(double) * sr.size()) == hipSuccess &&
          hipMalloc((void**)&c->rt_sc, sizeof(double) * sc.size()) == hipSuccess &&
          hipMalloc((void**)&c->R1_base, sizeof(double) * kMaxChains * np) == hipSuccess &&
@@ -271,6 +288,8 @@ int tdscha_ctx_create(tdscha_ctx** out, int n_modes, int n_configs, const double
          hipMalloc((void**)&c->rot_f, sizeof(double) * kMaxChains * np) == hipSuccess;
     ok = ok && hipMemcpy(c->rt_cols, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(c->rt_d, dd.data(), sizeof(int) * n_modes, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->rt_bs, bs.data(), sizeof(int) * n_modes, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(c->rt_pstart, pstart.data(), sizeof(int) * (n_modes + 2), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(c->rt_sr, sr.data(), sizeof(double) * sr.size(), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy(c->rt_sc, sc.data(), sizeof(double) * sc.size(), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && rot_setup(c, d_X, d_Y, d_rho) == 0 && hipStreamSynchronize(c->stream) == hipSuccess;
@@ -304,7 +323,7 @@ void tdscha_ctx_destroy(tdscha_ctx* ctx) {
                   c->tproj, c->cart_tmp, c->cart_tmp2, c->mode_tmp, c->psi_dev, c->psi_out_dev,
                   c->stage_in, c->stage_out, c->f_fin, c->d2v_fin, c->f_proj, c->epi,
                   c->coef32, c->wpart, c->zpart32, c->fpart_s, c->sbsc, c->pmax,
-                  c->rt_cols, c->rt_d, c->rt_sr, c->rt_sc, c->R1_base, c->Y1_base, c->rot_t1,
+                  c->rt_cols, c->rt_d, c->rt_bs, c->rt_pstart, c->rt_sr, c->rt_sc, c->R1_base, c->Y1_base, c->rot_t1,
                   c->rot_t2, c->rot_f};
   for (void* p : ptrs)
     if (p) hipFree(p);

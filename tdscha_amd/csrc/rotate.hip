@@ -186,11 +186,245 @@ __global__ void rot_accumulate(const double* __restrict__ T1, const double* __re
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)n * n) return;
   const int r = (int)(e / n), q = (int)(e % n);
-  for (int k = 0; k < g; ++k) {
-    const int ch = (p0 + k) / nsym;
-    d2v_all[(long)ch * n * n + e] += normd * T1[(long)k * nn + (long)r * np + q];
-    if (e < n) f_all[(long)ch * n + e] += normf * F1[(long)k * np + e];
+  // the pairs of one chain are consecutive: the running sums stay in
+  // registers and go back when the chain changes (same order of additions)
+  double t[kMaxChains], fv[kMaxChains];
+#pragma unroll
+  for (int k = 0; k < kMaxChains; ++k) {
+    const int kk = k < g ? k : g - 1;
+    t[k] = T1[(long)kk * nn + (long)r * np + q];
+    fv[k] = e < n ? F1[(long)kk * np + e] : 0.0;
   }
+  int ch = p0 / nsym;
+  double dv = d2v_all[(long)ch * n * n + e], fa = e < n ? f_all[(long)ch * n + e] : 0.0;
+#pragma unroll
+  for (int k = 0; k < kMaxChains; ++k) {
+    if (k < g) {
+      const int c = (p0 + k) / nsym;
+      if (c != ch) {
+        d2v_all[(long)ch * n * n + e] = dv;
+        if (e < n) f_all[(long)ch * n + e] = fa;
+        ch = c;
+        dv = d2v_all[(long)ch * n * n + e];
+        fa = e < n ? f_all[(long)ch * n + e] : 0.0;
+      }
+      dv += normd * t[k];
+      fa += normf * fv[k];
+    }
+  }
+  d2v_all[(long)ch * n * n + e] = dv;
+  if (e < n) f_all[(long)ch * n + e] = fa;
+}
+
+// ---- panel form, used when every block is a range of consecutive modes
+// listed in ascending order and has at most kRotPanelDmax modes: bs[m] is the
+// first mode of m's block.  The modes are cut into panels of whole blocks, at
+// most kRotPanel modes each (a last panel holds the padding n .. np - 1).
+// Because S_j is block diagonal, the (row panel, column panel) tile of
+// S_j^T A S_j (or S_j A S_j^T) is L M R with M that tile of A and L, R the
+// panels' diagonal parts of S_j (transposed as the side requires): two dense
+// 32 x 32 x 32 FP64 MFMA products per work-group, everything through LDS.
+// Grids: (npanel^2, g), 256 threads (4 waves, one 16 x 16 quadrant each).
+constexpr int kRotPanel = 32;
+constexpr int kRotPanelDmax = 24;
+constexpr int kRotLdP = kRotPanel + 1;
+
+struct Panels {
+  const int* start;   // [npanel + 1] first mode of each panel, then np
+  int npanel;         // panels, the padding panel (when np > n) last
+  int nreal;          // panels holding modes below n
+};
+
+// table entry t of mode m: S_j[m, bs(m) + t] (ROW) or S_j[bs(m) + t, m]
+template <bool ROW>
+__device__ __forceinline__ double rot_entry(const RotTab& tb, int j, int m, int t) {
+  return ROW ? tb.row_s(j, m, t) : tb.col_s(j, m, t);
+}
+
+// The sides of a tile: L[rr][a] = entry (a - bs(r) + r0) of row r = r0 + rr
+// and R[b][qq] = entry (b - bs(q) + q0) of column q = q0 + qq inside the
+// mode's block, zero elsewhere and past the tile (h rows, w columns): both
+// are zeroed, then (after a barrier) the block entries are scattered (the
+// tables hold zeros past a mode's block, so the loads take table rows and
+// modes clamped into range and need no test).
+__device__ __forceinline__ void panel_sides_zero(double* __restrict__ L, double* __restrict__ R) {
+  for (int e = threadIdx.x; e < kRotPanel * kRotLdP; e += 256) {
+    L[e] = 0.0;
+    R[e] = 0.0;
+  }
+}
+// C = A B for 32 x 32 LDS matrices (row stride kRotLdP); wave w computes the
+// quadrant rows 16 (w / 2) .., columns 16 (w % 2) ..; lane holds
+// C[16 (w / 2) + 4 r + lane / 16][16 (w % 2) + lane % 16] in acc[r]
+__device__ __forceinline__ v4d panel_mfma(const double* __restrict__ A,
+                                          const double* __restrict__ B) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i = 16 * (w >> 1) + (lane & 15), jc = 16 * (w & 1) + (lane & 15), kk = lane >> 4;
+  v4d acc = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int ks = 0; ks < kRotPanel / 4; ++ks)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(A[i * kRotLdP + 4 * ks + kk],
+                                               B[(4 * ks + kk) * kRotLdP + jc], acc, 0, 0, 0);
+  return acc;
+}
+
+// zeros on a tile that holds padding rows or columns (and the padding of the
+// vector output on the first column panel)
+__device__ __forceinline__ void panel_zero(int r0, int h, int q0, int w, int np, bool first_col,
+                                           double* __restrict__ out, double* __restrict__ vec) {
+  for (int e = threadIdx.x; e < h * w; e += 256) out[(long)(r0 + e / w) * np + q0 + e % w] = 0.0;
+  if (first_col)
+    for (int e = threadIdx.x; e < h; e += 256) vec[r0 + e] = 0.0;
+}
+
+// One tile of a rotation: out tile = L M R and, on the first column panel,
+// vec[r0 + rr] = sum_a L[rr][a] x[r0 + a].  In (OUT false): M = A (the
+// perturbation), L = S_r^T, R = S_c.  Out: M = D~ = fY A + (fY A)^T of the
+// raw sums A, L = S_r, R = S_c^T.  Every global load of a thread is issued
+// in one phase; the product L (M R) keeps M R in M's place.
+template <bool OUT>
+__device__ __forceinline__ void panel_run(const RotTab& tb, const int* __restrict__ bs,
+                                          const Panels& pn, int j, const double* __restrict__ A,
+                                          const double* __restrict__ fy,
+                                          const double* __restrict__ x, int np,
+                                          double* __restrict__ out, double* __restrict__ vec,
+                                          double* Ls, double* Ms, double* Rs, double* xs) {
+  const int pi = blockIdx.x / pn.npanel, pj = blockIdx.x % pn.npanel;
+  const int r0 = pn.start[pi], h = pn.start[pi + 1] - r0;
+  const int q0 = pn.start[pj], w = pn.start[pj + 1] - q0;
+  if (pi >= pn.nreal || pj >= pn.nreal) {
+    panel_zero(r0, h, q0, w, np, pj == 0, out, vec);
+    return;
+  }
+  const int tid = threadIdx.x;
+  constexpr int kM = kRotPanel * kRotPanel / 256;
+  constexpr int kU = kRotPanel * kRotPanelDmax / 256;
+  static_assert(kRotPanel * kRotPanelDmax == 256 * kU, "entries per thread");
+  // ---- loads: the tile (direct term, and the transposed term read along
+  // the rows of A), the side entries, the vector slice
+  // (rows and columns clamped into the tile, so every load is unconditional)
+  double mv[kM], mt[kM];
+#pragma unroll
+  for (int u = 0; u < kM; ++u) {
+    const int e = tid + 256 * u, a = e / kRotPanel, b = e % kRotPanel;
+    const int ar = a < h ? a : h - 1, bc = b < w ? b : w - 1;
+    mv[u] = A[(long)(r0 + ar) * np + q0 + bc];
+    if (OUT) mv[u] *= fy[r0 + ar];
+    if (!(a < h && b < w)) mv[u] = 0.0;
+    if (OUT) {
+      // transposed term of element (b, a): A[q0 + a][r0 + b]
+      const int aw = a < w ? a : w - 1, bh = b < h ? b : h - 1;
+      mt[u] = fy[q0 + aw] * A[(long)(q0 + aw) * np + r0 + bh];
+    }
+  }
+  double lv[kU], rv[kU];
+  int lb[kU], ld[kU], rb[kU], rd[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int e = tid + 256 * u, xx = e / kRotPanelDmax, t = e % kRotPanelDmax;
+    const int tt = t < tb.dmax ? t : tb.dmax - 1;
+    const int mr = r0 + (xx < h ? xx : h - 1), mc = q0 + (xx < w ? xx : w - 1);
+    lv[u] = rot_entry<OUT>(tb, j, mr, tt);
+    rv[u] = rot_entry<OUT>(tb, j, mc, tt);
+    lb[u] = bs[mr];
+    ld[u] = tb.d[mr];
+    rb[u] = bs[mc];
+    rd[u] = tb.d[mc];
+  }
+  const double xv0 = x[r0 + (tid < h ? tid : h - 1)];
+  const double xv = tid < h ? xv0 : 0.0;
+  panel_sides_zero(Ls, Rs);
+  __syncthreads();
+  // ---- stores
+#pragma unroll
+  for (int u = 0; u < kM; ++u) {
+    const int e = tid + 256 * u;
+    Ms[(e / kRotPanel) * kRotLdP + e % kRotPanel] = mv[u];
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const int e = tid + 256 * u, xx = e / kRotPanelDmax, t = e % kRotPanelDmax;
+    if (xx < h && t < ld[u]) Ls[xx * kRotLdP + lb[u] - r0 + t] = lv[u];
+    if (xx < w && t < rd[u]) Rs[(rb[u] - q0 + t) * kRotLdP + xx] = rv[u];
+  }
+  if (tid < kRotPanel) xs[tid] = xv;
+  __syncthreads();
+  if (OUT) {
+#pragma unroll
+    for (int u = 0; u < kM; ++u) {
+      const int e = tid + 256 * u, a = e / kRotPanel, b = e % kRotPanel;
+      if (b < h && a < w) Ms[b * kRotLdP + a] += mt[u];
+    }
+    __syncthreads();
+  }
+  // ---- products
+  const int lane = tid & 63, wv = tid >> 6;
+  const int i0 = 16 * (wv >> 1) + (lane >> 4), jc = 16 * (wv & 1) + (lane & 15);
+  const v4d t = panel_mfma(Ms, Rs);
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Ms[(i0 + 4 * r) * kRotLdP + jc] = t[r];
+  __syncthreads();
+  const v4d o = panel_mfma(Ls, Ms);
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+    if (i0 + 4 * r < h && jc < w) out[(long)(r0 + i0 + 4 * r) * np + q0 + jc] = o[r];
+  if (pj == 0 && tid < h) {
+    double v = 0.0;
+    for (int a = 0; a < h; ++a) v += Ls[tid * kRotLdP + a] * xs[a];
+    vec[r0 + tid] = v;
+  }
+}
+
+// Y1o[k] = S_j^T Y1[chain] S_j on one tile (zero on the padding); the tiles of
+// the first column panel also write R1o[k] = S_j^T R1[chain]
+__global__ __launch_bounds__(256) void rot_in_panel(RotTab tb, const int* __restrict__ bs,
+                                                    Panels pn, const double* __restrict__ Y1b,
+                                                    const double* __restrict__ R1b, int np,
+                                                    int nsym, int p0, double* __restrict__ R1o,
+                                                    double* __restrict__ Y1o) {
+  __shared__ double Ls[kRotPanel * kRotLdP], Ms[kRotPanel * kRotLdP], Rs[kRotPanel * kRotLdP];
+  __shared__ double xs[kRotPanel];
+  const int k = blockIdx.y, p = p0 + k, ch = p / nsym, j = p % nsym;
+  const long nn = (long)np * np;
+  panel_run<false>(tb, bs, pn, j, Y1b + (long)ch * nn, nullptr, R1b + (long)ch * np, np,
+                   Y1o + (long)k * nn, R1o + (long)k * np, Ls, Ms, Rs, xs);
+}
+
+// T1[k] = S_j D~ S_j^T on one tile, D~ = fY Z + (fY Z)^T of the pass's raw
+// outer sums Z = zsum[k] (zero on the padding); the tiles of the first column
+// panel also write F1[k] = S_j f~[k]
+__global__ __launch_bounds__(256) void rot_out_panel(RotTab tb, const int* __restrict__ bs,
+                                                     Panels pn, const double* __restrict__ zsum,
+                                                     const double* __restrict__ fy,
+                                                     const double* __restrict__ fsum, int np,
+                                                     int nsym, int p0, double* __restrict__ T1,
+                                                     double* __restrict__ F1) {
+  __shared__ double Ls[kRotPanel * kRotLdP], Ms[kRotPanel * kRotLdP], Rs[kRotPanel * kRotLdP];
+  __shared__ double xs[kRotPanel];
+  const int k = blockIdx.y, j = (p0 + k) % nsym;
+  const long nn = (long)np * np;
+  panel_run<true>(tb, bs, pn, j, zsum + (long)k * nn, fy, fsum + (long)k * np, np,
+                  T1 + (long)k * nn, F1 + (long)k * np, Ls, Ms, Rs, xs);
+}
+
+bool rot_panels_fit(int dmax) { return dmax <= kRotPanelDmax; }
+
+// panel starts for consecutive blocks: whole blocks, at most kRotPanel modes
+// per panel, then the padding n .. np - 1 as one more panel when np > n
+// (start has room for n + 2 entries)
+int rot_panels(const int* bs, const int* d, int n, int np, int* start, int* nreal) {
+  int np_ = 0, m = 0;
+  while (m < n) {
+    start[np_++] = m;
+    int e = m;
+    while (e < n && e + d[e] - m <= kRotPanel) e += d[e];
+    m = e > m ? e : m + d[m];   // a block wider than a panel is its own panel
+  }
+  *nreal = np_;
+  if (np > n) start[np_++] = n;
+  start[np_] = np;
+  return np_;
 }
 
 int rot_setup(Ctx* c, const double* X_dev, const double* Y_dev, const double* rho_dev) {
@@ -216,22 +450,34 @@ int run_pa_rot(Ctx* c, int nc, int apply_d4, int prec, double* f_dev, double* d2
   const int pairs = nc * ns;
   const double normf = 1.0 / ns, normd = 1.0 / (c->n_eff * ns);
   const unsigned eb = (unsigned)((nn + 255) / 256);
+  const Panels pn{c->rt_pstart, c->rt_npanel, c->rt_nreal};
+  const unsigned npanel2 = (unsigned)(pn.npanel * pn.npanel);
   for (int p0 = 0; p0 < pairs; p0 += kMaxChains) {
     const int g = pairs - p0 < kMaxChains ? pairs - p0 : kMaxChains;
-    hipLaunchKernelGGL(rot_in_right, dim3(eb, g), dim3(256), 0, s, tb, c->Y1_base, np, ns, p0,
-                       c->rot_t1);
-    hipLaunchKernelGGL(rot_in_left, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t1, c->R1_base, np,
-                       ns, p0, c->R1, c->Y1);
+    if (c->rt_panels) {
+      hipLaunchKernelGGL(rot_in_panel, dim3(npanel2, g), dim3(256), 0, s, tb, c->rt_bs, pn,
+                         c->Y1_base, c->R1_base, np, ns, p0, c->R1, c->Y1);
+    } else {
+      hipLaunchKernelGGL(rot_in_right, dim3(eb, g), dim3(256), 0, s, tb, c->Y1_base, np, ns, p0,
+                         c->rot_t1);
+      hipLaunchKernelGGL(rot_in_left, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t1, c->R1_base,
+                         np, ns, p0, c->R1, c->Y1);
+    }
     TDS_CHECK_HIP(hipGetLastError());
     const int rc = prec == PREC_FP64 ? launch_pa_fp64(c, g, apply_d4, s, nullptr)
                                      : launch_pa_split(c, g, apply_d4, prec, s, nullptr);
     if (rc) return rc;
-    hipLaunchKernelGGL(rot_sym, dim3(np / 32, np / 32, g), dim3(256), 0, s, c->d2v_out, c->fy, np,
-                       c->rot_t1);
-    hipLaunchKernelGGL(rot_out_right, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t1, np, ns, p0,
-                       c->rot_t2);
-    hipLaunchKernelGGL(rot_out_left, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t2, c->f_out, np, ns,
-                       p0, c->rot_t1, c->rot_f);
+    if (c->rt_panels) {
+      hipLaunchKernelGGL(rot_out_panel, dim3(npanel2, g), dim3(256), 0, s, tb, c->rt_bs, pn,
+                         c->d2v_out, c->fy, c->f_out, np, ns, p0, c->rot_t1, c->rot_f);
+    } else {
+      hipLaunchKernelGGL(rot_sym, dim3(np / 32, np / 32, g), dim3(256), 0, s, c->d2v_out, c->fy,
+                         np, c->rot_t1);
+      hipLaunchKernelGGL(rot_out_right, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t1, np, ns, p0,
+                         c->rot_t2);
+      hipLaunchKernelGGL(rot_out_left, dim3(eb, g), dim3(256), 0, s, tb, c->rot_t2, c->f_out, np,
+                         ns, p0, c->rot_t1, c->rot_f);
+    }
     hipLaunchKernelGGL(rot_accumulate, dim3((unsigned)(((long)n * n + 255) / 256)), dim3(256), 0, s,
                        c->rot_t1, c->rot_f, n, np, ns, p0, g, normf, normd, f_dev, d2v_dev);
     TDS_CHECK_HIP(hipGetLastError());

@@ -5,7 +5,9 @@ rotating the perturbation and the results (identity pinned on the oracle in
 tests/test_rotation_form.py).  Checked here against the C oracle's image
 average and against the image form on the device:
 * FP64 averages of 1-8 chains, with symmetry counts that make the 8-pair
-  passes span chains (n_syms = 6) and the SnTe space group (384);
+  passes span chains (n_syms = 6) and the SnTe space group (384), on blocks
+  of consecutive modes of up to 24 (tiled rotation kernels) and on blocks of
+  25 modes or of scattered modes (general rotation kernels);
 * split precision within its tolerance;
 * image ranges of whole configurations, and the error for partial ones;
 * whole L applications and device Lanczos chains (16-atom rock salt, 48
@@ -31,12 +33,38 @@ def snte():
     return load_snte()
 
 
+def _relabelled(s, seed):
+    """The same system with its modes relabelled by a random permutation, so
+    that its degenerate blocks are no longer runs of consecutive modes (the
+    general rotation kernels instead of the tiled ones)."""
+    n = s["X"].shape[1]
+    perm = np.random.default_rng(seed).permutation(n)   # mode m -> perm[m]
+    X, Y = np.empty_like(s["X"]), np.empty_like(s["Y"])
+    X[:, perm], Y[:, perm] = s["X"], s["Y"]
+    w = np.empty_like(s["w"])
+    w[perm] = s["w"]
+    sbi = np.empty_like(s["sbi"])
+    sbi[perm] = s["sbi"]
+    blocks = [perm[b].astype(np.int32) for b in s["blocks"]]
+    assert not all(np.all(np.diff(b) == 1) for b in blocks)
+    return dict(s, X=X, Y=Y, w=w, sbi=sbi, blocks=blocks)
+
+
 def _systems(snte):
     ms = snte["syms_full"]
     yield ("snte", snte["X"], snte["Y"], snte["w"], snte["rho"], float(snte["T"]),
            ms.symmetries, ms.degenerate_space, ms.sym_block_id)
     s = random_block_system(130, 700, 6, [3, 5, 2] + [1] * 120, seed=21)
     yield "random130", s["X"], s["Y"], s["w"], s["rho"], s["T"], s["syms"], s["blocks"], s["sbi"]
+    # blocks of 24 modes, the largest the tiled kernels take, next to a 25-mode
+    # block (general kernels)
+    s = random_block_system(200, 400, 3, [24, 1, 24, 7] + [24] * 6, seed=24)
+    yield "blocks24", s["X"], s["Y"], s["w"], s["rho"], s["T"], s["syms"], s["blocks"], s["sbi"]
+    s = random_block_system(200, 400, 3, [25, 1, 24, 6] + [24] * 6, seed=25)
+    yield "blocks25", s["X"], s["Y"], s["w"], s["rho"], s["T"], s["syms"], s["blocks"], s["sbi"]
+    s = _relabelled(random_block_system(130, 700, 6, [3, 5, 2] + [1] * 120, seed=26), 7)
+    yield ("random130-relabelled", s["X"], s["Y"], s["w"], s["rho"], s["T"], s["syms"],
+           s["blocks"], s["sbi"])
 
 
 def test_rotated_fp64_matches_oracle_and_images(snte):
