@@ -62,6 +62,9 @@ def _systems(snte):
     yield "blocks24", s["X"], s["Y"], s["w"], s["rho"], s["T"], s["syms"], s["blocks"], s["sbi"]
     s = random_block_system(200, 400, 3, [25, 1, 24, 6] + [24] * 6, seed=25)
     yield "blocks25", s["X"], s["Y"], s["w"], s["rho"], s["T"], s["syms"], s["blocks"], s["sbi"]
+    # n a multiple of 128: no padding panel
+    s = random_block_system(128, 400, 4, [24] * 5 + [5, 3], seed=27)
+    yield "n128", s["X"], s["Y"], s["w"], s["rho"], s["T"], s["syms"], s["blocks"], s["sbi"]
     s = _relabelled(random_block_system(130, 700, 6, [3, 5, 2] + [1] * 120, seed=26), 7)
     yield ("random130-relabelled", s["X"], s["Y"], s["w"], s["rho"], s["T"], s["syms"],
            s["blocks"], s["sbi"])
