@@ -135,6 +135,7 @@ struct Ctx {
   void* Y1h = nullptr;      // [kMaxChains, np, np] (16-bit)
   void* Y1l = nullptr;
   double* ysc = nullptr;    // [kMaxChains] power-of-two scale of split Y1
+  double* ysc3 = nullptr;   // [kMaxChains] scale of the folded fY R1 column
   double* bsc = nullptr;    // [kMaxChains, np] column scale of split B
   double* cmax = nullptr;   // [kMaxChains, 2] max|c1|, max|c2| (device)
   double* pmax = nullptr;   // [kMaxChains, kPartBlocks, 2] per-block maxima of the split prep

@@ -317,7 +317,7 @@ void tdscha_ctx_destroy(tdscha_ctx* ctx) {
   Ctx* c = reinterpret_cast<Ctx*>(ctx);
   if (c->stream) hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->U, c->F, c->rho_img, c->fy, c->fpsi, c->w, c->R1, c->Y1, c->coef, c->zpart,
-                  c->fpart, c->f_out, c->d2v_out, c->Y1h, c->Y1l, c->ysc, c->bsc, c->cmax,
+                  c->fpart, c->f_out, c->d2v_out, c->Y1h, c->Y1l, c->ysc, c->ysc3, c->bsc, c->cmax,
                   c->pair_ca, c->pair_cb, c->pair_sm, c->pair_sp, c->pair_harm_a, c->pair_harm_b,
                   c->pair_row, c->pair_col, c->std_coef, c->inv_coef, c->yw, c->reaw, c->pols, c->inv_perms, c->rep_dof, c->jrep, c->ju, c->rep_sum,
                   c->tproj, c->cart_tmp, c->cart_tmp2, c->mode_tmp, c->psi_dev, c->psi_out_dev,

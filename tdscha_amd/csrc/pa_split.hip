@@ -222,21 +222,44 @@ __global__ void ymax_kernel(const double* __restrict__ pmax, int nblocks, double
   if (threadIdx.x == 0) ysc[chain] = pow2_scale(red[0]);
 }
 
-// YB[c] block (column tile, K step) holds Y1[c][j][k] sy / su[k] split hi/lo;
-// per-chain epilogue column factors (fY R1, R1) in FP32.  One block per
-// (column j, chain).
+// When the modes are padded (n < np), the row-dot s3 = u . (fY R1) rides the
+// weights GEMM: the last padding column of YB holds fY R1 / su with a scale
+// of its own per chain (sy3: max_k |fY R1 / su| sy3 < 2^12), and the
+// accumulator of that column is s3 sy3 (the padding of U and F keeps it out
+// of the other row-dots).
+__global__ void foldmax_kernel(const double* __restrict__ R1all, const double* __restrict__ su,
+                               const double* __restrict__ fy, int np, double* __restrict__ ysc3) {
+  const int chain = blockIdx.x;
+  __shared__ double red[256];
+  double mx = 0.0;
+  for (int k = threadIdx.x; k < np; k += blockDim.x)
+    mx = fmax(mx, fabs(fy[k] * R1all[(long)chain * np + k] / su[k]));
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ysc3[chain] = pow2_scale(red[0]);
+}
+
+// YB[c] block (column tile, K step) holds Y1[c][j][k] sy / su[k] split hi/lo
+// (and, with ysc3 given, fY R1 sy3 / su in the last column); per-chain
+// epilogue column factors (fY R1, R1) in FP32.  One block per (column j,
+// chain).
 template <typename H>
 __global__ void prep_y(const double* __restrict__ Y1all, const double* __restrict__ R1all,
                        const double* __restrict__ su, const double* __restrict__ fy, int np,
-                       const double* __restrict__ ysc, u16* __restrict__ YBh,
-                       u16* __restrict__ YBl, float* __restrict__ epw) {
+                       const double* __restrict__ ysc, const double* __restrict__ ysc3,
+                       u16* __restrict__ YBh, u16* __restrict__ YBl, float* __restrict__ epw) {
   const int j = blockIdx.x, chain = blockIdx.y;
   const double* Y = Y1all + ((long)chain * np + j) * np;
-  const double sy = ysc[chain];
+  const bool fold = ysc3 != nullptr && j == np - 1;
+  const double sy = fold ? ysc3[chain] : ysc[chain];
   const int nct = np / kBR, nkc = np / kBK;
   const long base = ((long)chain * nct + j / kBR) * nkc * kBE;
   for (int k = threadIdx.x; k < np; k += blockDim.x) {
-    const double v = Y[k] * sy / su[k];
+    const double v = (fold ? fy[k] * R1all[(long)chain * np + k] : Y[k]) * sy / su[k];
     const u16 h = from_d<H>(v);
     const long a = base + (long)(k / kBK) * kBE + blk_off(j % kBR, k % kBK);
     YBh[a] = h;
@@ -274,7 +297,7 @@ __global__ void colw_kernel(const double* __restrict__ su, const double* __restr
 template <int K>
 using ic = std::integral_constant<int, K>;
 
-template <typename H, int NC>
+template <typename H, int NC, bool FOLD>
 __global__ __launch_bounds__(256, 1) void weights_split(
     const u16* __restrict__ UAh, const u16* __restrict__ UAl, const float* __restrict__ FA,
     const float* __restrict__ UA32,
@@ -463,19 +486,29 @@ __global__ __launch_bounds__(256, 1) void weights_split(
           for (int t = 0; t < 4; ++t) g[t] = f[t] * fp4[t];
 #pragma unroll
           for (int c = 0; c < NC; ++c) {
-            const float4 fr = *reinterpret_cast<const float4*>(cw + (3 + 2 * c) * kBR + col);
             const float4 rr = *reinterpret_cast<const float4*>(cw + (4 + 2 * c) * kBR + col);
-            const float fr4[4] = {fr.x, fr.y, fr.z, fr.w};
             const float rr4[4] = {rr.x, rr.y, rr.z, rr.w};
+            float fr4[4] = {0.f, 0.f, 0.f, 0.f};
+            if constexpr (!FOLD) {
+              const float4 fr = *reinterpret_cast<const float4*>(cw + (3 + 2 * c) * kBR + col);
+              fr4[0] = fr.x; fr4[1] = fr.y; fr4[2] = fr.z; fr4[3] = fr.w;
+            }
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
               const float v = acc[c][i][4 * q + t];
               p32[c][0] += v * u[t];
               p32[c][1] += v * g[t];
-              p32[c][2] += u[t] * fr4[t];
+              if constexpr (!FOLD) p32[c][2] += u[t] * fr4[t];
               p32[c][3] += f[t] * rr4[t];
             }
           }
+        }
+        if constexpr (FOLD) {
+          // s3 sy3 of the row: the folded column, tile column 127 of the last
+          // column tile (wave 3, q = 3, lane half 1, t = 3)
+          const bool holder = ct == nct - 1 && wave == 3 && lh == 1;
+#pragma unroll
+          for (int c = 0; c < NC; ++c) p32[c][2] = holder ? acc[c][i][15] : 0.f;
         }
         double p[NC][4];
 #pragma unroll
@@ -514,7 +547,8 @@ __global__ __launch_bounds__(256, 1) void weights_split(
 // coefB layout: [Mp/16][kMaxChains][4][16] floats (c1, c2, alpha, beta of
 // the 16 images of a K step, all chains of a step contiguous).
 __global__ void finalize_coef(const double* __restrict__ part, const double* __restrict__ rho_img,
-                              const double* __restrict__ ysc, long Mp, int nct, int apply_d4,
+                              const double* __restrict__ ysc, const double* __restrict__ ysc3,
+                              long Mp, int nct, int apply_d4,
                               long row_lo, long row_hi, double* __restrict__ coef_all,
                               float* __restrict__ coefB, double* __restrict__ pmax) {
   const int chain = blockIdx.y;
@@ -529,6 +563,7 @@ __global__ void finalize_coef(const double* __restrict__ part, const double* __r
     }
     s[0] *= isy;
     s[1] *= isy;
+    if (ysc3) s[2] /= ysc3[chain];   // the folded column carries its own scale
     // images outside [row_lo, row_hi) carry no weight (image-range calls)
     const double rho = (row >= row_lo && row < row_hi) ? rho_img[row] : 0.0;
     const double c1 = rho * (-s[2] / 3.0 + apply_d4 * s[0] / 8.0);
@@ -927,6 +962,7 @@ int ensure_split(Ctx* c, int prec) {
     TDS_CHECK_HIP(hipMalloc(&c->Y1l, sizeof(u16) * kMaxChains * (size_t)np * np));
     TDS_CHECK_HIP(hipMalloc((void**)&c->epi, sizeof(float) * kMaxChains * 2 * np));
     TDS_CHECK_HIP(hipMalloc((void**)&c->ysc, sizeof(double) * kMaxChains));
+    TDS_CHECK_HIP(hipMalloc((void**)&c->ysc3, sizeof(double) * kMaxChains));
     TDS_CHECK_HIP(hipMalloc((void**)&c->cmax, sizeof(double) * kMaxChains * 2));
     TDS_CHECK_HIP(hipMalloc((void**)&c->pmax, sizeof(double) * kMaxChains * kPartBlocks * 2));
     TDS_CHECK_HIP(hipMalloc((void**)&c->coef32, sizeof(float) * kMaxChains * Mp * 4));
@@ -961,7 +997,7 @@ int ensure_split(Ctx* c, int prec) {
   return 0;
 }
 
-template <typename H, int NC>
+template <typename H, int NC, bool FOLD>
 static void launch_weights(Ctx* c, const SplitStore& st, int chain0, hipStream_t s) {
   const int np = c->np;
   const int nct = np / kBR;
@@ -969,7 +1005,7 @@ static void launch_weights(Ctx* c, const SplitStore& st, int chain0, hipStream_t
   // persistent: at most 32 work-groups per XCD, none without a tile
   const int per_xcd_tiles = (nrt + 7) / 8 * nct;
   const int nper = per_xcd_tiles < 32 ? per_xcd_tiles : 32;
-  hipLaunchKernelGGL((weights_split<H, NC>), dim3((unsigned)(8 * nper)), dim3(256), 0, s,
+  hipLaunchKernelGGL((weights_split<H, NC, FOLD>), dim3((unsigned)(8 * nper)), dim3(256), 0, s,
                      (const u16*)st.Uh, (const u16*)st.Ul, st.FA, st.UA32,
                      (const u16*)c->Y1h, (const u16*)c->Y1l, st.colw, c->epi, np, c->Mp, chain0,
                      nrt, c->wpart);
@@ -993,15 +1029,28 @@ static int launch_all(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* e
   hipLaunchKernelGGL(ymax_part_kernel, dim3(kYmaxBlocks, nc), dim3(256), 0, s, c->Y1, st.su, np,
                      c->pmax);
   hipLaunchKernelGGL(ymax_kernel, dim3(nc), dim3(64), 0, s, c->pmax, kYmaxBlocks, c->ysc);
+  // with padded modes the row-dot s3 rides the GEMM in the last padding column
+  const bool fold = c->n < np;
+  if (fold)
+    hipLaunchKernelGGL(foldmax_kernel, dim3(nc), dim3(256), 0, s, c->R1, st.su, c->fy, np,
+                       c->ysc3);
   hipLaunchKernelGGL(prep_y<H>, dim3(np, nc), dim3(256), 0, s, c->Y1, c->R1, st.su, c->fy, np,
-                     c->ysc, (u16*)c->Y1h, (u16*)c->Y1l, c->epi);
+                     c->ysc, fold ? c->ysc3 : nullptr, (u16*)c->Y1h, (u16*)c->Y1l, c->epi);
   // chains in groups of up to 3 per pass
   for (int c0 = 0; c0 < nc;) {
     const int g = nc - c0 >= 3 ? 3 : nc - c0;
-    switch (g) {
-      case 3: launch_weights<H, 3>(c, st, c0, s); break;
-      case 2: launch_weights<H, 2>(c, st, c0, s); break;
-      default: launch_weights<H, 1>(c, st, c0, s); break;
+    if (fold) {
+      switch (g) {
+        case 3: launch_weights<H, 3, true>(c, st, c0, s); break;
+        case 2: launch_weights<H, 2, true>(c, st, c0, s); break;
+        default: launch_weights<H, 1, true>(c, st, c0, s); break;
+      }
+    } else {
+      switch (g) {
+        case 3: launch_weights<H, 3, false>(c, st, c0, s); break;
+        case 2: launch_weights<H, 2, false>(c, st, c0, s); break;
+        default: launch_weights<H, 1, false>(c, st, c0, s); break;
+      }
     }
     c0 += g;
   }
@@ -1009,7 +1058,8 @@ static int launch_all(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* e
   const long fin_blocks_l = (Mp + 255) / 256;
   const int fin_blocks = fin_blocks_l < kPartBlocks ? (int)fin_blocks_l : kPartBlocks;
   hipLaunchKernelGGL(finalize_coef, dim3((unsigned)fin_blocks, nc), dim3(256), 0, s, c->wpart,
-                     c->rho_img, c->ysc, Mp, nct, apply_d4, c->row_lo, c->row_hi,
+                     c->rho_img, c->ysc, fold ? c->ysc3 : nullptr, Mp, nct, apply_d4, c->row_lo,
+                     c->row_hi,
                      (double*)nullptr, c->coef32, c->pmax);
   hipLaunchKernelGGL(bscale_kernel, dim3(nc), dim3(256), 0, s, c->pmax, fin_blocks, st.umax,
                      st.fmax, c->fy, np, c->cmax, c->sbsc);
