@@ -21,6 +21,8 @@ def rel(a, b):
 @pytest.mark.parametrize("n,N,blocks,nsym", [
     (45, 2000, [1] * 45, 1),
     (130, 300, [3, 5, 2] + [1] * 120, 6),
+    # n a multiple of 128: no padding column, the row-dot u.(fY R1) in the epilogue
+    (128, 600, [4, 4] + [1] * 120, 3),
 ])
 def test_split_perturbed_averages_error(prec, n, N, blocks, nsym):
     from oracle import c_oracle as CO
