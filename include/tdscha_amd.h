@@ -73,9 +73,9 @@ int tdscha_set_ksplit(tdscha_ctx* ctx, int ksplit);
  * Autotuner of the real-space perturbed averages (no counterpart in the
  * reference): times one pass of n_chains (1..8) perturbations at the given
  * precision for each K split around the default (multiples of 8; reps
- * passes each), keeps the fastest for that precision and returns it in
- * ksplit_out with its time in best_ms.  Every K split computes the same sums
- * in another order.
+ * passes each, in two sweeps, the smaller time of a candidate counting),
+ * keeps the fastest for that precision and returns it in ksplit_out with its
+ * time in best_ms.  Every K split computes the same sums in another order.
  */
 int tdscha_autotune(tdscha_ctx* ctx, int n_chains, int precision, int reps, int* ksplit_out,
                     double* best_ms);
