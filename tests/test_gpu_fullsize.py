@@ -15,7 +15,8 @@
   in is tested;
 - linearity: L(s x + t y) = s L x + t L y in FP64 (relative 1e-12 of |L|);
 - the split FP16/BF16 hi/lo products against FP64: relative L2 error of L x
-  at most 1e-5 (the spectrum tolerance of the split path is 1e-4).
+  at most 1e-5 (the spectrum tolerance of the split path is 1e-4);
+- the operator-rotation form against the image form (FP64 and fp16x2).
 """
 import os
 
@@ -118,3 +119,29 @@ def test_full_size_split_against_fp64(full, prec):
     err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
     print(prec, "relative L2 error of L x:", err)
     assert err.max() < 1e-5
+
+
+def test_full_size_rotation_form_equals_the_image_form(full):
+    """The operator-rotation form at the full size (20000 configurations, the
+    48 point-group symmetries applied to the perturbation and the results;
+    blocks of up to 24 consecutive modes, the panel kernels) against the image
+    form, whose FP64 averages the oracle pins above: FP64 to 1e-12, fp16x2
+    within the split tolerance of one perturbed average (2e-5)."""
+    from tdscha_amd.engine import Engine
+    lanc, _, _ = full
+    n = lanc.n_modes
+    rot = Engine(lanc.X, lanc.Y, lanc.w, lanc.rho, lanc.T, lanc.symmetries,
+                 lanc.degenerate_space, lanc.sym_block_id, rotate=True)
+    assert rot.n_images == 960000
+    rng = np.random.default_rng(23)
+    R1 = rng.standard_normal((3, n))
+    Y1 = rng.standard_normal((3, n, n)) * 1e-3
+    Y1 = Y1 + Y1.transpose(0, 2, 1)
+    fi, di = lanc.engine.perturb_averages_batched(R1, Y1, 1)
+    fr, dr = rot.perturb_averages_batched(R1, Y1, 1)
+    print("rotation vs images FP64: f %.2e d2v %.2e" % (rel(fr, fi), rel(dr, di)))
+    assert rel(fr, fi) < 1e-12 and rel(dr, di) < 1e-12
+    fs, ds = rot.perturb_averages_batched(R1, Y1, 1, "fp16x2")
+    print("rotation fp16x2 vs images FP64: f %.2e d2v %.2e" % (rel(fs, fi), rel(ds, di)))
+    assert rel(fs, fi) < 2e-5 and rel(ds, di) < 2e-5
+    rot.close()
