@@ -222,8 +222,8 @@ __global__ void ymax_kernel(const double* __restrict__ pmax, int nblocks, double
   if (threadIdx.x == 0) ysc[chain] = pow2_scale(red[0]);
 }
 
-// When the modes are padded (n < np), the row-dot s3 = u . (fY R1) rides the
-// weights GEMM: the last padding column of YB holds fY R1 / su with a scale
+// When the modes are padded (n < np), the fp16 path lets the row-dot
+// s3 = u . (fY R1) ride the weights GEMM: the last padding column of YB holds fY R1 / su with a scale
 // of its own per chain (sy3: max_k |fY R1 / su| sy3 < 2^12), and the
 // accumulator of that column is s3 sy3 (the padding of U and F keeps it out
 // of the other row-dots).
@@ -1029,8 +1029,11 @@ static int launch_all(Ctx* c, int nc, int apply_d4, hipStream_t s, hipEvent_t* e
   hipLaunchKernelGGL(ymax_part_kernel, dim3(kYmaxBlocks, nc), dim3(256), 0, s, c->Y1, st.su, np,
                      c->pmax);
   hipLaunchKernelGGL(ymax_kernel, dim3(nc), dim3(64), 0, s, c->pmax, kYmaxBlocks, c->ysc);
-  // with padded modes the row-dot s3 rides the GEMM in the last padding column
-  const bool fold = c->n < np;
+  // with padded modes the row-dot s3 rides the fp16 GEMM in the last padding
+  // column (hi/lo products good to ~2^-22, like the FP32 sums they replace);
+  // bf16 hi/lo carries ~2^-17, and s3 dominates c1, so the bf16 path keeps
+  // the FP32 row-dot (its spectrum error would grow fivefold)
+  const bool fold = c->n < np && std::is_same<H, _Float16>::value;
   if (fold)
     hipLaunchKernelGGL(foldmax_kernel, dim3(nc), dim3(256), 0, s, c->R1, st.su, c->fy, np,
                        c->ysc3);
