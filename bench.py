@@ -283,6 +283,62 @@ def rotation_run(ens, psis, lanc_img, results, args):
     return out
 
 
+def terminator_run(n_pot=8, ref_steps=1000, k_max=120):
+    """The packaged learned terminators (one ensemble per chain length K) on
+    fresh held-out potentials of their training family (16-atom rock salt,
+    k3 ~ U(-4, 0), k4 ~ U(2, 10), T ~ U(50, 600) K, seeds outside the training
+    streams): per K, the median L-inf spectrum error of the standard and the
+    learned closure after K steps against the 1000-step FP64 chain, and the
+    steps the standard terminator needs to reach the learned one's error."""
+    from tdscha_amd.lanczos import Lanczos, continued_fraction
+    from tdscha_amd.rocksalt import Ensemble, RockSalt
+    from tdscha_amd.terminator import TerminatorFamily
+    w = np.linspace(0.0, 600.0, 2000) / RY_TO_CM
+    eta = 5.0 / RY_TO_CM
+    rng = np.random.default_rng(4242)
+    chains = []
+    t0 = time.perf_counter()
+    for i in range(n_pot):
+        k3, k4, T = rng.uniform(-4, 0), rng.uniform(2, 10), rng.uniform(50, 600)
+        ens = Ensemble(RockSalt(L=2, k3=k3, k4=k4), T, 2000, seed=8_000_000 + i)
+        lanc = Lanczos(ens)
+        lanc.init(use_symmetries=False)
+        psis = []
+        for pv in ((1, 0, 0), (0, 1, 0), (0, 0, 1)):
+            lanc.prepare_perturbation(ens.ir_vector(pv), masses_exp=-1)
+            psis.append(lanc.psi.copy())
+        out, _ = lanc.run_FT_batched(np.array(psis), ref_steps, reorth=True, precision="fp64")
+        lanc.close()
+        chains += [(a, b) for a, b in out if len(a) >= k_max]
+    fam = TerminatorFamily.packaged()
+    res = {"potentials": n_pot, "chains": len(chains), "ref_steps": ref_steps,
+           "grid": "2000 points 0-600 cm^-1, smearing 5 cm^-1"}
+    for K in fam.lengths:
+        e_std, e_l, k_eq, used = [], [], [], 0
+        for a, b in chains:
+            ref = -np.imag(continued_fraction(a, b, b, w, True, smearing=eta))
+            sc = np.abs(ref).max()
+
+            def err_std(k):
+                g = continued_fraction(a[:k], b[:k], b[:k], w, True, smearing=eta)
+                return float(np.abs(-np.imag(g) - ref).max() / sc)
+
+            el = float(np.abs(-np.imag(fam.green_function(a[:K], b[:K], w, eta)) - ref).max() / sc)
+            used += fam.last_used == "learned"
+            e_std.append(err_std(K))
+            e_l.append(el)
+            k_eq.append(next((k for k in range(K, k_max + 1) if err_std(k) <= el), k_max + 1))
+        med = float(np.median(k_eq))
+        res["K%d" % K] = {"median_err_standard": float(np.median(e_std)),
+                          "median_err_learned": float(np.median(e_l)),
+                          "median_steps_standard_needs": med,
+                          "fewer_L_applications_median": 1.0 - K / med,
+                          "learned_used": int(used)}
+    fam.close()
+    res["wall_s"] = time.perf_counter() - t0
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +367,8 @@ def main():
     ap.add_argument("--no-rotation", action="store_true",
                     help="skip the operator-rotation form's measurement")
     ap.add_argument("--rotation-steps", type=int, default=5)
+    ap.add_argument("--no-terminator", action="store_true",
+                    help="skip the learned terminator's held-out evaluation")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (the ensemble sum "
@@ -502,6 +560,13 @@ def main():
             log("q-space: %s" % {k: v for k, v in qres.items() if k != "workload"})
         except Exception as e:          # reported, never hides the headline line
             qres = {"error": str(e)}
+    term = None
+    if not args.no_terminator:
+        try:
+            term = terminator_run()
+            log("learned terminator: %s" % term)
+        except Exception as e:          # reported, never hides the headline line
+            term = {"error": str(e)}
     cpu = cpu_all = None
     if not args.no_cpu:
         cpu, cpu_all = cpu_baseline(ens, lanc, 64, args.cpu_budget)
@@ -544,6 +609,7 @@ def main():
         "cpu_baseline_all_cores": cpu_all,
         "operator_rotation": rot,
         "qspace": qres,
+        "learned_terminator": term,
     }
     print(json.dumps(out), flush=True)
     lanc.close()
